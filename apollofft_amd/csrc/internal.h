@@ -1,0 +1,181 @@
+// internal.h -- shared host/device definitions of libapollo_nufft_hip.
+//
+// Data layout in HBM (all row-major, z fastest, like ndarray Array3):
+//   fine grid   float2[mx][my][mz]
+//   modes       float2[nx][ny][nz]   (FFT order, k <= n/2 positive)
+//   points      float coords[a][j*stride], float2 values[j]
+// The kernel phi is evaluated from per-tap polynomials: for a point at grid
+// coordinate t, the first tap is i0 = ceil(t - w), x1 = i0 - t in [-w,-w+1),
+// z = 2(x1 + w) - 1 in [-1,1), and tap i has phi(x1 + i) = P_i(z).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace anufft {
+
+constexpr int kMinW = 2;
+constexpr int kMaxW = 8;            // largest half-width with device kernels
+constexpr int kMaxTaps = 2 * kMaxW; // non-zero taps per axis
+constexpr int kPolyCoefs = 11;      // degree-10 per-tap polynomials
+constexpr int kRadixBits = 8;       // LSD counting-sort digit width
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;       // points per thread per chunk
+constexpr int kSortChunk = kSortThreads * kSortItems;
+
+// Per-tap kernel polynomials, passed by value (kernarg segment -> scalar regs).
+// c[d][i] multiplies z^d for tap i; dc[d][i] is the same for d phi/dx.
+struct KbPoly {
+  float c[kPolyCoefs][kMaxTaps];
+  float dc[kPolyCoefs][kMaxTaps];
+};
+
+// Geometry of the oversampled grid and the spreading tiles.
+struct Geom {
+  int m[3];          // oversampled sizes (1D: m[1] = m[2] = 1)
+  int bin[3];        // tile size in fine cells
+  int nbin[3];       // tiles per axis
+  int pad[3];        // padded tile extents bin + 2w
+  double scale[3];   // m / L: grid units per unit length
+};
+
+struct SortBuffers {
+  uint32_t *keys[2];
+  uint32_t *idx[2];
+  uint32_t *cnt;       // kRadix * nchunks digit counts, then offsets
+  uint32_t *scan_tmp;  // block sums of the scan
+  uint32_t *bin_start; // nbins + 1
+  uint32_t *sub_ofs;   // nbins + 1 (exclusive scan of subproblem counts)
+  int4 *subs;          // (bin, start, count, 0)
+  int nbins;
+  int nchunks;
+  int passes;
+  int max_sub;
+};
+
+// One batched FFT pass over lines of a 3D (or 1D) array.
+struct FftPass {
+  const float2 *in;
+  float2 *out;
+  int L;       // transform length
+  int nin;     // inputs per line (nin < L: kept modes, zero padded)
+  int nout;    // outputs per line (nout < L: kept modes only)
+  int64_t nlines, nb;               // line l -> (a = l / nb, b = l % nb)
+  int64_t in_sa, in_sb, in_si;      // element strides
+  int64_t out_sa, out_sb, out_si;
+  const double *dec_in;  // factor per input index (nin) or null
+  const double *dec_out; // factor per output index (nout) or null
+  double scale;
+  int inverse;           // 0: exp(-i), 1: exp(+i) (unnormalized)
+  const float2 *tw;      // exp(-2 pi i k / L), k in [0, L)
+  int nfac;
+  int fac[16];
+  int lb;                // lines per block
+};
+
+// Arguments shared by the 3D spread / interp kernels.
+struct PointArgs {
+  const float *x, *y, *z;
+  int64_t stride;
+  int64_t num_points;
+  const uint32_t *perm;      // sorted order -> original point index
+  const int4 *subs;
+  const uint32_t *nsub;      // device count of subproblems
+};
+
+// ---- launchers (defined in the .hip files) ---------------------------------
+hipError_t launch_fill(float2 *p, int64_t n, hipStream_t s);
+
+hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
+                       hipStream_t s);
+
+hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
+
+hipError_t launch_spread3d(int w, const PointArgs &pa, const float2 *values, float2 *grid,
+                           const Geom &g, const KbPoly &poly, int num_blocks, hipStream_t s);
+hipError_t launch_interp3d(int w, const PointArgs &pa, const float2 *grid, float2 *out,
+                           const Geom &g, const KbPoly &poly, int num_blocks, hipStream_t s);
+hipError_t launch_grad3d(int w, const PointArgs &pa, const float2 *grid, const float2 *u,
+                         float *gx, float *gy, float *gz, const Geom &g, const KbPoly &poly,
+                         int num_blocks, hipStream_t s);
+
+hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t num_points,
+                           const float2 *values, float2 *grid, const Geom &g,
+                           const KbPoly &poly, hipStream_t s);
+hipError_t launch_interp1d(int w, const float *x, int64_t stride, int64_t num_points,
+                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+                           hipStream_t s);
+hipError_t launch_grad1d(int w, const float *x, int64_t stride, int64_t num_points,
+                         const float2 *grid, const float2 *u, float *gx, const Geom &g,
+                         const KbPoly &poly, hipStream_t s);
+
+// LDS bytes of the 3D spread/interp kernels for a geometry.
+size_t spread_lds_bytes(int w, const Geom &g);
+size_t interp_lds_bytes(int w, const Geom &g, bool grad);
+
+// ---- device helpers ----------------------------------------------------------
+#if defined(__HIPCC__)
+// Fold a coordinate into [0, m) grid units and return the cell index.
+// t = x * scale, t -= floor(t/m) m, t += m if negative; g = min(floor(t), m-1).
+// q*m is exact (integer times integer below 2^53), so the subtraction rounds
+// once whether or not it is contracted into an FMA; the CPU replay of the
+// binning repeats this sequence in IEEE double and gets identical cells.
+__device__ __forceinline__ double fold_coord(float xf, double scale, int m, int &cell) {
+  double t = (double)xf * scale;
+  double md = (double)m;
+  double q = floor(t / md);
+  t = t - q * md;
+  if (t < 0.0) t = t + md;
+  int c = (int)floor(t);
+  if (c > m - 1) c = m - 1;
+  if (c < 0) c = 0;
+  cell = c;
+  return t;
+}
+
+// Evaluate the 2w non-zero tap weights of one axis for grid coordinate t.
+// Returns the first tap index i0 = ceil(t - w).
+template <int W>
+__device__ __forceinline__ int kb_taps(double t, const KbPoly &P, float *wt) {
+  double i0d = ceil(t - (double)W);
+  float z = (float)(2.0 * (i0d - t + (double)W) - 1.0);
+#pragma unroll
+  for (int i = 0; i < 2 * W; ++i) {
+    float v = P.c[kPolyCoefs - 1][i];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, P.c[d][i]);
+    wt[i] = v;
+  }
+  if (z <= -1.0f) wt[0] = 0.0f; // x1 == -w: phi(-w) = 0 (kaiser_bessel.rs:111-112)
+  return (int)i0d;
+}
+
+// Value and derivative (d phi / d x, per grid unit) taps.
+template <int W>
+__device__ __forceinline__ int kb_taps_deriv(double t, const KbPoly &P, float *wt, float *dwt) {
+  double i0d = ceil(t - (double)W);
+  float z = (float)(2.0 * (i0d - t + (double)W) - 1.0);
+#pragma unroll
+  for (int i = 0; i < 2 * W; ++i) {
+    float v = P.c[kPolyCoefs - 1][i];
+    float dv = P.dc[kPolyCoefs - 1][i];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) {
+      v = fmaf(v, z, P.c[d][i]);
+      dv = fmaf(dv, z, P.dc[d][i]);
+    }
+    wt[i] = v;
+    dwt[i] = dv;
+  }
+  if (z <= -1.0f) {
+    wt[0] = 0.0f;
+    dwt[0] = 0.0f;
+  }
+  return (int)i0d;
+}
+#endif
+
+} // namespace anufft

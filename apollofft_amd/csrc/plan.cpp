@@ -1,0 +1,1015 @@
+// plan.cpp -- plans, KB metadata, workspace layout and the C ABI of
+// libapollo_nufft_hip (declared in include/apollo_nufft.h).
+//
+// Plan creation follows NufftPlan1D::new / NufftPlan3D::new
+// (apollo-nufft/src/application/execution/transform/dimension_1d.rs:127-154,
+// dimension_3d.rs:193-232): same oversampled length rule, same beta with the
+// integer sigma, same FP64 deconvolution factors.  In addition it builds the
+// device tables the kernels need: FP64 deconvolution factors, FP64-built
+// twiddles rounded to FP32, and per-tap polynomial fits of the KB kernel and
+// its derivative.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/apollo_nufft.h"
+#include "internal.h"
+
+using namespace anufft;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+anufft_status fail(anufft_status st, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return st;
+}
+
+anufft_status hip_fail(hipError_t e, const char *what) {
+  return fail(ANUFFT_HIP_ERROR, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ---- KB primitives in FP64 (kaiser_bessel.rs:55-164) ------------------------
+double kb_i0(double z) {
+  double scaled = 0.25 * z * z, sum = 1.0, term = 1.0;
+  for (int k = 1; k <= 256; ++k) {
+    double kf = (double)k;
+    term *= scaled / (kf * kf);
+    sum += term;
+    if (term <= 2.220446049250313e-16 * sum) break;
+  }
+  return sum;
+}
+
+double kb_i1(double z) {
+  // I1(z) = sum_k (z/2)^(2k+1) / (k! (k+1)!)
+  double half = 0.5 * z, q = half * half, term = half, sum = half;
+  for (int k = 1; k <= 256; ++k) {
+    term *= q / ((double)k * (double)(k + 1));
+    sum += term;
+    if (std::fabs(term) <= 2.220446049250313e-16 * std::fabs(sum)) break;
+  }
+  return sum;
+}
+
+double kb_phi(double x, double w, double beta, double i0b) {
+  double u2 = (x / w) * (x / w);
+  if (u2 >= 1.0) return 0.0;
+  return kb_i0(beta * std::sqrt(1.0 - u2)) / i0b;
+}
+
+double kb_dphi(double x, double w, double beta, double i0b) {
+  double u2 = (x / w) * (x / w);
+  if (u2 >= 1.0) return 0.0;
+  double s = std::sqrt(1.0 - u2);
+  if (s < 1e-300) return 0.0;
+  return kb_i1(beta * s) * beta * (-x / (w * w * s)) / i0b;
+}
+
+double kb_phi_hat(double xi, int w, double beta, double i0b) {
+  double a = 2.0 * M_PI * (double)w * xi;
+  double z2 = beta * beta - a * a;
+  double prefix = 2.0 * (double)w / i0b;
+  if (std::fabs(z2) < 1e-30) return prefix;
+  if (z2 > 0.0) {
+    double s = std::sqrt(z2);
+    return prefix * std::sinh(s) / s;
+  }
+  double s = std::sqrt(-z2);
+  return prefix * std::sin(s) / s;
+}
+
+int64_t signed_index(int64_t k, int64_t n) { return k <= n / 2 ? k : k - n; }
+
+// Chebyshev interpolation of f on [-1,1] at kPolyCoefs nodes, returned as
+// monomial coefficients in z (FP64).
+template <typename F>
+void cheb_fit(F f, double *mono) {
+  const int N = kPolyCoefs;
+  double fz[N], a[N];
+  for (int k = 0; k < N; ++k) fz[k] = f(std::cos(M_PI * (k + 0.5) / N));
+  for (int j = 0; j < N; ++j) {
+    double s = 0.0;
+    for (int k = 0; k < N; ++k) s += fz[k] * std::cos(M_PI * j * (k + 0.5) / N);
+    a[j] = s * 2.0 / N;
+  }
+  a[0] *= 0.5;
+  // T_j in the monomial basis by T_{j+1} = 2 z T_j - T_{j-1}
+  double Tm1[N] = {0}, T0[N] = {0}, T1[N] = {0};
+  T0[0] = 1.0;
+  for (int d = 0; d < N; ++d) mono[d] = 0.0;
+  for (int j = 0; j < N; ++j) {
+    for (int d = 0; d < N; ++d) mono[d] += a[j] * T0[d];
+    // next: T_1 = z, then T_{j+1} = 2 z T_j - T_{j-1}
+    const double mult = j == 0 ? 1.0 : 2.0;
+    for (int d = 0; d < N; ++d) T1[d] = -Tm1[d] + (d > 0 ? mult * T0[d - 1] : 0.0);
+    for (int d = 0; d < N; ++d) {
+      Tm1[d] = T0[d];
+      T0[d] = T1[d];
+    }
+  }
+}
+
+double horner_f32(const float c[kPolyCoefs][kMaxTaps], int tap, double z) {
+  double v = c[kPolyCoefs - 1][tap];
+  for (int d = kPolyCoefs - 2; d >= 0; --d) v = v * z + (double)c[d][tap];
+  return v;
+}
+
+// Stockham factor order: radix 8 first, then 4/2, 3, 5, other primes.
+std::vector<int> factorize(int L) {
+  std::vector<int> f;
+  int n = L;
+  while (n % 8 == 0) {
+    f.push_back(8);
+    n /= 8;
+  }
+  if (n % 4 == 0) {
+    f.push_back(4);
+    n /= 4;
+  }
+  if (n % 2 == 0) {
+    f.push_back(2);
+    n /= 2;
+  }
+  for (int p = 3; n > 1 && p * p <= n; p += 2)
+    while (n % p == 0) {
+      f.push_back(p);
+      n /= p;
+    }
+  if (n > 1) f.push_back(n);
+  return f;
+}
+
+size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct Layout {
+  size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
+         sub_ofs = 0, subs = 0, grid = 0, fftbuf = 0, total = 0;
+  int nchunks = 0;
+};
+
+} // namespace
+
+struct anufft_plan {
+  int dim = 3;
+  int64_t n[3] = {1, 1, 1};
+  double d[3] = {1, 1, 1};
+  double L[3] = {1, 1, 1};
+  int64_t m[3] = {1, 1, 1};
+  int sigma = 2, w = 6;
+  double beta = 0, i0b = 1;
+  anufft_opts opts{};
+  int device = 0;
+  int cus = 256;
+  std::vector<double> deconv[3];
+  double *d_deconv[3] = {nullptr, nullptr, nullptr};
+  float2 *d_tw[3] = {nullptr, nullptr, nullptr};
+  std::vector<int> fac[3];
+  KbPoly poly{};
+  double poly_err = 0;
+  Geom geom{};
+  int nbins = 1;
+  int passes = 1;
+  int max_sub = 1024;
+  int lds_max = 163840; // gfx950 LDS per workgroup
+  hipStream_t host_stream = nullptr;
+
+  int64_t modes() const { return n[0] * n[1] * n[2]; }
+  int64_t cells() const { return m[0] * m[1] * m[2]; }
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (dev >= 0 && dev != prev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+Layout layout(const anufft_plan *p, int64_t M) {
+  Layout l;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += align_up(bytes);
+    return o;
+  };
+  if (p->dim == 3) {
+    int64_t nch = M > 0 ? (M + kSortChunk - 1) / kSortChunk : 1;
+    l.nchunks = (int)nch;
+    size_t Mp = (size_t)std::max<int64_t>(M, 1);
+    l.keys0 = take(4 * Mp);
+    l.keys1 = take(4 * Mp);
+    l.idx0 = take(4 * Mp);
+    l.idx1 = take(4 * Mp);
+    int64_t cnt_n = (int64_t)kRadix * nch;
+    l.cnt = take(4 * (size_t)cnt_n);
+    int64_t scan_n = std::max<int64_t>(cnt_n, p->nbins + 1);
+    l.scan = take(4 * (size_t)((scan_n + 1023) / 1024 + 1));
+    l.bin_start = take(4 * (size_t)(p->nbins + 1));
+    l.sub_ofs = take(4 * (size_t)(p->nbins + 1));
+    int64_t nsub = p->nbins + (M + p->max_sub - 1) / p->max_sub + 1;
+    l.subs = take(16 * (size_t)nsub);
+    l.grid = take(8 * (size_t)p->cells());
+    l.fftbuf = take(8 * (size_t)(p->m[0] * p->m[1] * p->n[2]));
+  } else {
+    l.grid = take(8 * (size_t)p->m[0]);
+  }
+  l.total = off;
+  return l;
+}
+
+anufft_status build_tables(anufft_plan *p) {
+  // deconvolution 1/phi_hat(k_signed/m) (kaiser_bessel.rs:137-148)
+  for (int a = 0; a < p->dim; ++a) {
+    p->deconv[a].resize(p->n[a]);
+    for (int64_t k = 0; k < p->n[a]; ++k) {
+      double xi = (double)signed_index(k, p->n[a]) / (double)p->m[a];
+      p->deconv[a][k] = 1.0 / kb_phi_hat(xi, p->w, p->beta, p->i0b);
+    }
+  }
+  // per-tap polynomials of phi and d phi/dx on x = -w + i + (z+1)/2
+  const int T = 2 * p->w;
+  const double wd = (double)p->w;
+  double mono[kPolyCoefs];
+  for (int i = 0; i < T; ++i) {
+    cheb_fit([&](double z) { return kb_phi(-wd + i + 0.5 * (z + 1.0), wd, p->beta, p->i0b); },
+             mono);
+    for (int d = 0; d < kPolyCoefs; ++d) p->poly.c[d][i] = (float)mono[d];
+    cheb_fit([&](double z) { return kb_dphi(-wd + i + 0.5 * (z + 1.0), wd, p->beta, p->i0b); },
+             mono);
+    for (int d = 0; d < kPolyCoefs; ++d) p->poly.dc[d][i] = (float)mono[d];
+  }
+  double err = 0.0;
+  for (int i = 0; i < T; ++i)
+    for (int s = 1; s < 256; ++s) {
+      double z = -1.0 + 2.0 * s / 256.0;
+      double x = -wd + i + 0.5 * (z + 1.0);
+      err = std::max(err, std::fabs(horner_f32(p->poly.c, i, z) - kb_phi(x, wd, p->beta, p->i0b)));
+    }
+  p->poly_err = err;
+  for (int a = 0; a < p->dim; ++a) p->fac[a] = factorize((int)p->m[a]);
+  if (p->device < 0) return ANUFFT_OK;
+
+  DeviceGuard guard(p->device);
+  hipError_t es = hipStreamCreateWithFlags(&p->host_stream, hipStreamNonBlocking);
+  if (es != hipSuccess) return hip_fail(es, "hipStreamCreate");
+  for (int a = 0; a < p->dim; ++a) {
+    hipError_t e = hipMalloc(&p->d_deconv[a], sizeof(double) * p->n[a]);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(deconv)");
+    e = hipMemcpy(p->d_deconv[a], p->deconv[a].data(), sizeof(double) * p->n[a],
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(deconv)");
+    int L = (int)p->m[a];
+    std::vector<float2> tw(L);
+    for (int k = 0; k < L; ++k) {
+      double ang = -2.0 * M_PI * (double)k / (double)L;
+      tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    e = hipMalloc(&p->d_tw[a], sizeof(float2) * L);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
+    e = hipMemcpy(p->d_tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(twiddles)");
+  }
+  return ANUFFT_OK;
+}
+
+void destroy_plan(anufft_plan *p) {
+  if (!p) return;
+  if (p->device >= 0) {
+    DeviceGuard guard(p->device);
+    for (int a = 0; a < 3; ++a) {
+      if (p->d_deconv[a]) (void)hipFree(p->d_deconv[a]);
+      if (p->d_tw[a]) (void)hipFree(p->d_tw[a]);
+    }
+    if (p->host_stream) (void)hipStreamDestroy(p->host_stream);
+  }
+  delete p;
+}
+
+// A plan records its device; without any HIP device the plan still carries
+// its KB metadata (host tables) and execute calls report
+// ANUFFT_DEVICE_UNAVAILABLE.
+void init_device(anufft_plan *p) {
+  int count = 0;
+  p->device = -1;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return;
+  int dev = p->opts.device_id;
+  if (dev < 0 && hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev >= count) return;
+  p->device = dev;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+    p->cus = v;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+    p->lds_max = v;
+}
+
+// Tile sizes: largest cube edge whose spread, interp and gradient kernels fit
+// the LDS of one workgroup, clipped to the grid.
+void choose_bins(anufft_plan *p) {
+  Geom &g = p->geom;
+  for (int a = 0; a < 3; ++a) {
+    g.m[a] = (int)p->m[a];
+    g.scale[a] = (double)p->m[a] / p->L[a];
+  }
+  if (p->dim == 1) {
+    for (int a = 0; a < 3; ++a) {
+      g.bin[a] = g.m[a];
+      g.nbin[a] = 1;
+      g.pad[a] = g.m[a];
+    }
+    p->nbins = 1;
+    p->passes = 1;
+    return;
+  }
+  const int cand[] = {32, 24, 16, 12, 8, 6, 4, 2, 1};
+  int user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
+  for (int c : cand) {
+    for (int a = 0; a < 3; ++a) {
+      int b = user ? p->opts.bin_size[a] : c;
+      g.bin[a] = std::min(b, g.m[a]);
+      g.nbin[a] = (g.m[a] + g.bin[a] - 1) / g.bin[a];
+      g.pad[a] = g.bin[a] + 2 * p->w;
+    }
+    size_t need = std::max(spread_lds_bytes(p->w, g),
+                           std::max(interp_lds_bytes(p->w, g, false), interp_lds_bytes(p->w, g, true)));
+    if (user || need <= (size_t)p->lds_max) break;
+  }
+  p->nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
+  int bits = 1;
+  while ((1ll << bits) < (long long)p->nbins) ++bits;
+  p->passes = (bits + kRadixBits - 1) / kRadixBits;
+  p->max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 1024;
+}
+
+anufft_status finish_create(anufft_plan *p, anufft_plan **out) {
+  init_device(p);
+  if (p->opts.device_id >= 0 && p->device != p->opts.device_id) {
+    int id = p->opts.device_id;
+    destroy_plan(p);
+    return fail(ANUFFT_DEVICE_UNAVAILABLE, "device_id %d is not available", id);
+  }
+  p->beta = M_PI * (1.0 - 1.0 / (2.0 * (double)p->sigma)) * (double)(2 * p->w);
+  p->i0b = kb_i0(p->beta);
+  choose_bins(p);
+  if (p->dim == 3) {
+    size_t need = std::max(spread_lds_bytes(p->w, p->geom),
+                           std::max(interp_lds_bytes(p->w, p->geom, false),
+                                    interp_lds_bytes(p->w, p->geom, true)));
+    if (need > (size_t)p->lds_max) {
+      destroy_plan(p);
+      return fail(ANUFFT_INVALID_ARG, "bin_size needs %zu bytes of LDS, device has %d", need,
+                  p->lds_max);
+    }
+  }
+  for (int a = 0; a < p->dim; ++a) {
+    // one LDS line must hold the oversampled length (two buffers, 64 KiB)
+    if (p->m[a] > 4096) {
+      destroy_plan(p);
+      return fail(ANUFFT_UNSUPPORTED, "oversampled length %lld exceeds the in-LDS FFT limit 4096",
+                  (long long)p->m[a]);
+    }
+  }
+  anufft_status st = build_tables(p);
+  if (st != ANUFFT_OK) {
+    destroy_plan(p);
+    return st;
+  }
+  *out = p;
+  return ANUFFT_OK;
+}
+
+anufft_status check_common(int32_t sigma, int32_t w) {
+  if (sigma < 2) return fail(ANUFFT_INVALID_ARG, "sigma must be >= 2 (got %d)", sigma);
+  if (w < 2) return fail(ANUFFT_INVALID_ARG, "kernel_width must be >= 2 (got %d)", w);
+  if (w > kMaxW)
+    return fail(ANUFFT_UNSUPPORTED, "kernel_width %d exceeds the device maximum %d", w, kMaxW);
+  return ANUFFT_OK;
+}
+
+int64_t over_len(int64_t n, int sigma, int w, int policy, int dim) {
+  int64_t m = n * sigma;
+  if (dim == 1 && policy == ANUFFT_GRID_REFERENCE) return m; // dimension_1d.rs:131
+  m = std::max<int64_t>(m, 2 * w + 1);
+  if (policy == ANUFFT_GRID_EXACT) return m;
+  int64_t q = 1;
+  while (q < m) q <<= 1; // dimension_3d.rs:48-54
+  return q;
+}
+
+// ---- FFT pass construction ---------------------------------------------------
+FftPass make_pass(const anufft_plan *p, int axis, const float2 *in, float2 *out, int nin,
+                  int nout, int64_t na, int64_t nb, int64_t in_sa, int64_t in_sb, int64_t in_si,
+                  int64_t out_sa, int64_t out_sb, int64_t out_si, bool inverse, double scale) {
+  FftPass f{};
+  f.in = in;
+  f.out = out;
+  f.L = (int)p->m[axis];
+  f.nin = nin;
+  f.nout = nout;
+  f.nlines = na * nb;
+  f.nb = nb;
+  f.in_sa = in_sa;
+  f.in_sb = in_sb;
+  f.in_si = in_si;
+  f.out_sa = out_sa;
+  f.out_sb = out_sb;
+  f.out_si = out_si;
+  f.dec_in = inverse ? p->d_deconv[axis] : nullptr;
+  f.dec_out = inverse ? nullptr : p->d_deconv[axis];
+  f.scale = scale;
+  f.inverse = inverse ? 1 : 0;
+  f.tw = p->d_tw[axis];
+  f.nfac = (int)p->fac[axis].size();
+  for (int i = 0; i < f.nfac && i < 16; ++i) f.fac[i] = p->fac[axis][i];
+  // lines per block: two LDS buffers of lb*L complex within 64 KiB, and
+  // enough blocks to cover the chip when there are few lines.
+  int lb = std::max(1, 4096 / f.L);
+  while (lb > 8 && (f.nlines + lb - 1) / lb < 2 * (int64_t)p->cus) lb >>= 1;
+  f.lb = (int)std::max<int64_t>(1, std::min<int64_t>(lb, f.nlines));
+  return f;
+}
+
+struct Exec {
+  const anufft_plan *p;
+  hipStream_t s;
+  char *ws;
+  Layout lay;
+  float2 *grid() const { return (float2 *)(ws + lay.grid); }
+  float2 *fftbuf() const { return (float2 *)(ws + lay.fftbuf); }
+  SortBuffers sort_buffers() const {
+    SortBuffers sb{};
+    sb.keys[0] = (uint32_t *)(ws + lay.keys0);
+    sb.keys[1] = (uint32_t *)(ws + lay.keys1);
+    sb.idx[0] = (uint32_t *)(ws + lay.idx0);
+    sb.idx[1] = (uint32_t *)(ws + lay.idx1);
+    sb.cnt = (uint32_t *)(ws + lay.cnt);
+    sb.scan_tmp = (uint32_t *)(ws + lay.scan);
+    sb.bin_start = (uint32_t *)(ws + lay.bin_start);
+    sb.sub_ofs = (uint32_t *)(ws + lay.sub_ofs);
+    sb.subs = (int4 *)(ws + lay.subs);
+    sb.nbins = p->nbins;
+    sb.nchunks = lay.nchunks;
+    sb.passes = p->passes;
+    sb.max_sub = p->max_sub;
+    return sb;
+  }
+  PointArgs point_args(int64_t M, const float *const c[3], int64_t stride) const {
+    PointArgs pa{};
+    pa.x = c[0];
+    pa.y = p->dim == 3 ? c[1] : c[0];
+    pa.z = p->dim == 3 ? c[2] : c[0];
+    pa.stride = stride;
+    pa.num_points = M;
+    SortBuffers sb = sort_buffers();
+    pa.perm = sb.idx[p->passes & 1];
+    pa.subs = sb.subs;
+    pa.nsub = sb.sub_ofs + p->nbins;
+    return pa;
+  }
+  int spread_blocks(int64_t M) const {
+    int64_t up = (int64_t)p->nbins + (M + p->max_sub - 1) / p->max_sub;
+    int64_t b = std::min<int64_t>(up, 4 * (int64_t)p->cus);
+    return (int)std::max<int64_t>(b, 1);
+  }
+
+  // T1 grid -> modes: forward passes z, y, x with crop + deconvolution.
+  hipError_t grid_to_modes(float2 *out) const {
+    const int64_t mx = p->m[0], my = p->m[1], mz = p->m[2];
+    const int64_t nx = p->n[0], ny = p->n[1], nz = p->n[2];
+    float2 *g = grid(), *a1 = fftbuf();
+    hipError_t e;
+    if (p->dim == 1) {
+      FftPass f = make_pass(p, 0, g, out, (int)mx, (int)nx, 1, 1, 0, 0, 1, 0, 0, 1, false, 1.0);
+      return launch_fft_pass(f, s);
+    }
+    // z: grid[mx][my][mz] -> a1[mx][my][nz]
+    FftPass fz = make_pass(p, 2, g, a1, (int)mz, (int)nz, mx, my, my * mz, mz, 1, my * nz, nz, 1,
+                           false, 1.0);
+    if ((e = launch_fft_pass(fz, s)) != hipSuccess) return e;
+    // y: a1[mx][my][nz] -> a2[mx][ny][nz] (in the grid buffer)
+    FftPass fy = make_pass(p, 1, a1, g, (int)my, (int)ny, mx, nz, my * nz, 1, nz, ny * nz, 1, nz,
+                           false, 1.0);
+    if ((e = launch_fft_pass(fy, s)) != hipSuccess) return e;
+    // x: a2[mx][ny][nz] -> out[nx][ny][nz]
+    FftPass fx = make_pass(p, 0, g, out, (int)mx, (int)nx, ny, nz, nz, 1, ny * nz, nz, 1, ny * nz,
+                           false, 1.0);
+    return launch_fft_pass(fx, s);
+  }
+
+  // T2 modes -> grid: deconvolve, zero pad, inverse passes x, y, z.
+  hipError_t modes_to_grid(const float2 *modes, bool compat) const {
+    const int64_t mx = p->m[0], my = p->m[1], mz = p->m[2];
+    const int64_t nx = p->n[0], ny = p->n[1], nz = p->n[2];
+    float2 *g = grid(), *c = fftbuf();
+    hipError_t e;
+    if (p->dim == 1) {
+      FftPass f = make_pass(p, 0, modes, g, (int)nx, (int)mx, 1, 1, 0, 0, 1, 0, 0, 1, true, 1.0);
+      return launch_fft_pass(f, s);
+    }
+    double scale = compat ? 1.0 / ((double)mx * (double)my * (double)mz) : 1.0;
+    // x: modes[nx][ny][nz] -> b[mx][ny][nz] (in the grid buffer)
+    FftPass fx = make_pass(p, 0, modes, g, (int)nx, (int)mx, ny, nz, nz, 1, ny * nz, nz, 1, ny * nz,
+                           true, 1.0);
+    if ((e = launch_fft_pass(fx, s)) != hipSuccess) return e;
+    // y: b[mx][ny][nz] -> c[mx][my][nz]
+    FftPass fy = make_pass(p, 1, g, c, (int)ny, (int)my, mx, nz, ny * nz, 1, nz, my * nz, 1, nz,
+                           true, 1.0);
+    if ((e = launch_fft_pass(fy, s)) != hipSuccess) return e;
+    // z: c[mx][my][nz] -> grid[mx][my][mz]
+    FftPass fz = make_pass(p, 2, c, g, (int)nz, (int)mz, mx, my, my * nz, nz, 1, my * mz, mz, 1,
+                           true, scale);
+    return launch_fft_pass(fz, s);
+  }
+};
+
+anufft_status check_exec(const anufft_plan *p, int64_t M, const float *const coords[3],
+                         const void *a, const void *b, void *ws, size_t ws_bytes, Layout &lay) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  if (M < 0) return fail(ANUFFT_INVALID_ARG, "num_points must be >= 0");
+  if (M > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points %lld exceeds u32", (long long)M);
+  if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
+    return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
+  if ((M > 0 && !a) || !b) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
+  if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  lay = layout(p, M);
+  if (!ws || ws_bytes < lay.total)
+    return fail(ANUFFT_CAPACITY_EXCEEDED, "workspace of %zu bytes is smaller than the %zu required",
+                ws_bytes, lay.total);
+  return ANUFFT_OK;
+}
+
+} // namespace
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+void anufft_default_opts(anufft_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->grid_policy = ANUFFT_GRID_REFERENCE;
+  o->compat_reference_type2_3d_scaling = 0;
+  o->device_id = -1;
+}
+
+const char *anufft_last_error(void) { return g_last_error.c_str(); }
+
+const char *anufft_version(void) { return "apollo-nufft-hip 0.1.0 (gfx950)"; }
+
+anufft_status anufft_plan1d_create(int64_t n, double dx, int32_t sigma, int32_t kernel_width,
+                                   const anufft_opts *opts, anufft_plan **plan) {
+  if (!plan) return fail(ANUFFT_INVALID_ARG, "plan out-pointer is NULL");
+  *plan = nullptr;
+  // UniformDomain1D::new (grid.rs:36-48)
+  if (n <= 0) return fail(ANUFFT_INVALID_ARG, "invalid NUFFT metadata field n: %lld (must be > 0)", (long long)n);
+  if (!std::isfinite(dx) || dx <= 0.0)
+    return fail(ANUFFT_INVALID_ARG, "invalid NUFFT metadata field dx: %g (must be finite and > 0)", dx);
+  anufft_status st = check_common(sigma, kernel_width);
+  if (st != ANUFFT_OK) return st;
+  if (n > (int64_t)UINT32_MAX / sigma) return fail(ANUFFT_U32_OVERFLOW, "oversampled length overflows u32");
+  anufft_plan *p = new (std::nothrow) anufft_plan();
+  if (!p) return fail(ANUFFT_INVALID_ARG, "out of host memory");
+  if (opts) p->opts = *opts;
+  else anufft_default_opts(&p->opts);
+  p->dim = 1;
+  p->n[0] = n;
+  p->d[0] = dx;
+  p->L[0] = (double)n * dx;
+  p->sigma = sigma;
+  p->w = kernel_width;
+  p->m[0] = over_len(n, sigma, kernel_width, p->opts.grid_policy, 1);
+  return finish_create(p, plan);
+}
+
+anufft_status anufft_plan3d_create(int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                                   double dz, int32_t sigma, int32_t kernel_width,
+                                   const anufft_opts *opts, anufft_plan **plan) {
+  if (!plan) return fail(ANUFFT_INVALID_ARG, "plan out-pointer is NULL");
+  *plan = nullptr;
+  // UniformGrid3D::new (grid.rs:76-110)
+  const int64_t nn[3] = {nx, ny, nz};
+  const double dd[3] = {dx, dy, dz};
+  const char *nf[3] = {"nx", "ny", "nz"}, *df[3] = {"dx", "dy", "dz"};
+  for (int a = 0; a < 3; ++a)
+    if (nn[a] <= 0)
+      return fail(ANUFFT_INVALID_ARG, "invalid NUFFT metadata field %s: %lld (must be > 0)", nf[a], (long long)nn[a]);
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(dd[a]) || dd[a] <= 0.0)
+      return fail(ANUFFT_INVALID_ARG, "invalid NUFFT metadata field %s: %g (must be finite and > 0)", df[a], dd[a]);
+  anufft_status st = check_common(sigma, kernel_width);
+  if (st != ANUFFT_OK) return st;
+  anufft_plan *p = new (std::nothrow) anufft_plan();
+  if (!p) return fail(ANUFFT_INVALID_ARG, "out of host memory");
+  if (opts) p->opts = *opts;
+  else anufft_default_opts(&p->opts);
+  p->dim = 3;
+  p->sigma = sigma;
+  p->w = kernel_width;
+  for (int a = 0; a < 3; ++a) {
+    p->n[a] = nn[a];
+    p->d[a] = dd[a];
+    p->L[a] = (double)nn[a] * dd[a];
+    p->m[a] = over_len(nn[a], sigma, kernel_width, p->opts.grid_policy, 3);
+  }
+  // device.rs:893-898: dimensions and the grid size must fit in u32; the
+  // kernels index the grid with 32-bit cells per axis and 64-bit offsets.
+  if ((double)p->m[0] * (double)p->m[1] * (double)p->m[2] > 4294967295.0) {
+    delete p;
+    return fail(ANUFFT_U32_OVERFLOW, "oversampled grid size exceeds u32");
+  }
+  return finish_create(p, plan);
+}
+
+anufft_status anufft_plan_destroy(anufft_plan *plan) {
+  destroy_plan(plan);
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_plan_get_meta(const anufft_plan *p, anufft_meta *meta) {
+  if (!p || !meta) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  std::memset(meta, 0, sizeof(*meta));
+  meta->dim = p->dim;
+  for (int a = 0; a < 3; ++a) {
+    meta->n[a] = p->n[a];
+    meta->m[a] = p->m[a];
+    meta->length[a] = a < p->dim ? p->L[a] : 0.0;
+    meta->bin_size[a] = p->geom.bin[a];
+  }
+  meta->kernel_width = p->w;
+  meta->sigma = p->sigma;
+  meta->beta = p->beta;
+  meta->i0_beta = p->i0b;
+  meta->grid_policy = p->opts.grid_policy;
+  meta->compat_reference_type2_3d_scaling = p->opts.compat_reference_type2_3d_scaling;
+  meta->max_subproblem = p->max_sub;
+  meta->poly_degree = kPolyCoefs - 1;
+  meta->poly_max_err = p->poly_err;
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_plan_get_deconv(const anufft_plan *p, int32_t axis, double *out_n) {
+  if (!p || !out_n) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  if (axis < 0 || axis >= p->dim) return fail(ANUFFT_INVALID_ARG, "axis %d out of range", axis);
+  std::memcpy(out_n, p->deconv[axis].data(), sizeof(double) * p->deconv[axis].size());
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_workspace_size(const anufft_plan *p, int64_t max_points, size_t *bytes) {
+  if (!p || !bytes) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  if (max_points < 0) return fail(ANUFFT_INVALID_ARG, "max_points must be >= 0");
+  *bytes = layout(p, max_points).total;
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float *const coords[3],
+                                   int64_t stride, const float *values, float *out, void *ws,
+                                   size_t ws_bytes, void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, values, out, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  hipError_t e = launch_fill(ex.grid(), p->cells(), ex.s);
+  if (e != hipSuccess) return hip_fail(e, "fill");
+  if (M > 0) {
+    if (p->dim == 3) {
+      PointArgs pa = ex.point_args(M, coords, stride);
+      e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+      if (e != hipSuccess) return hip_fail(e, "binning");
+      e = launch_spread3d(p->w, pa, (const float2 *)values, ex.grid(), p->geom, p->poly,
+                          ex.spread_blocks(M), ex.s);
+    } else {
+      e = launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(), p->geom,
+                          p->poly, ex.s);
+    }
+    if (e != hipSuccess) return hip_fail(e, "spread");
+  }
+  e = ex.grid_to_modes((float2 *)out);
+  if (e != hipSuccess) return hip_fail(e, "fft");
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float *const coords[3],
+                                   int64_t stride, const float *modes, float *out, void *ws,
+                                   size_t ws_bytes, void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, out, modes, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  if (M == 0) return ANUFFT_OK;
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  hipError_t e;
+  if (p->dim == 3) {
+    PointArgs pa = ex.point_args(M, coords, stride);
+    e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+    if (e != hipSuccess) return hip_fail(e, "binning");
+    e = ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
+    if (e != hipSuccess) return hip_fail(e, "fft");
+    e = launch_interp3d(p->w, pa, ex.grid(), (float2 *)out, p->geom, p->poly,
+                        ex.spread_blocks(M), ex.s);
+  } else {
+    e = ex.modes_to_grid((const float2 *)modes, false);
+    if (e != hipSuccess) return hip_fail(e, "fft");
+    e = launch_interp1d(p->w, coords[0], stride, M, ex.grid(), (float2 *)out, p->geom, p->poly,
+                        ex.s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "interp");
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
+                                         const float *const coords[3], int64_t stride,
+                                         const float *u, const float *modes_h,
+                                         float *const grad[3], void *ws, size_t ws_bytes,
+                                         void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, u, modes_h, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  if (!grad || !grad[0]) return fail(ANUFFT_INVALID_ARG, "grad[0] is NULL");
+  if (M == 0) return ANUFFT_OK;
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  hipError_t e;
+  // the derivative always uses the unnormalized (direct-sum) scaling
+  if (p->dim == 3) {
+    PointArgs pa = ex.point_args(M, coords, stride);
+    e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+    if (e != hipSuccess) return hip_fail(e, "binning");
+    e = ex.modes_to_grid((const float2 *)modes_h, false);
+    if (e != hipSuccess) return hip_fail(e, "fft");
+    e = launch_grad3d(p->w, pa, ex.grid(), (const float2 *)u, grad[0], grad[1], grad[2], p->geom,
+                      p->poly, ex.spread_blocks(M), ex.s);
+  } else {
+    e = ex.modes_to_grid((const float2 *)modes_h, false);
+    if (e != hipSuccess) return hip_fail(e, "fft");
+    e = launch_grad1d(p->w, coords[0], stride, M, ex.grid(), (const float2 *)u, grad[0], p->geom,
+                      p->poly, ex.s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "grad");
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_debug_num_bins(const anufft_plan *p, int64_t *num_bins) {
+  if (!p || !num_bins) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  *num_bins = p->nbins;
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_debug_sort(const anufft_plan *p, int64_t M, const float *const coords[3],
+                                int64_t stride, uint32_t *perm, uint32_t *bin_start, void *ws,
+                                size_t ws_bytes, void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, perm, bin_start, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  if (p->dim != 3) return fail(ANUFFT_UNSUPPORTED, "1D plans do not bin points");
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  PointArgs pa = ex.point_args(M, coords, stride);
+  SortBuffers sb = ex.sort_buffers();
+  hipError_t e = sort_points(pa, p->geom, 3, sb, ex.s);
+  if (e != hipSuccess) return hip_fail(e, "binning");
+  if (M > 0 && (e = hipMemcpyAsync(perm, pa.perm, 4 * (size_t)M, hipMemcpyDeviceToDevice, ex.s)) != hipSuccess)
+    return hip_fail(e, "copy perm");
+  if ((e = hipMemcpyAsync(bin_start, sb.bin_start, 4 * (size_t)(p->nbins + 1), hipMemcpyDeviceToDevice, ex.s)) != hipSuccess)
+    return hip_fail(e, "copy bin_start");
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_debug_spread(const anufft_plan *p, int64_t M, const float *const coords[3],
+                                  int64_t stride, const float *values, float *grid, void *ws,
+                                  size_t ws_bytes, void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, values, grid, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  hipError_t e = launch_fill((float2 *)grid, p->cells(), ex.s);
+  if (e != hipSuccess) return hip_fail(e, "fill");
+  if (M == 0) return ANUFFT_OK;
+  if (p->dim == 3) {
+    PointArgs pa = ex.point_args(M, coords, stride);
+    e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+    if (e != hipSuccess) return hip_fail(e, "binning");
+    e = launch_spread3d(p->w, pa, (const float2 *)values, (float2 *)grid, p->geom, p->poly,
+                        ex.spread_blocks(M), ex.s);
+  } else {
+    e = launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, (float2 *)grid,
+                        p->geom, p->poly, ex.s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "spread");
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32_t nprob,
+                                          const int64_t *max_points, size_t *bytes) {
+  if (!plans || !max_points || !bytes || nprob < 0) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  size_t tot = 0;
+  for (int i = 0; i < nprob; ++i) {
+    if (!plans[i]) return fail(ANUFFT_INVALID_ARG, "plan %d is NULL", i);
+    if (plans[i]->dim != 3) return fail(ANUFFT_INVALID_ARG, "batched execution takes 3D plans");
+    tot += align_up(layout(plans[i], max_points[i]).total);
+  }
+  *bytes = tot;
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int32_t nprob,
+                                           const int64_t *num_points, const float *const *coords,
+                                           int64_t stride, const float *const *values,
+                                           float *const *out, void *ws, size_t ws_bytes,
+                                           void *stream) {
+  size_t need = 0;
+  anufft_status st = anufft_batch_workspace_size(plans, nprob, num_points, &need);
+  if (st != ANUFFT_OK) return st;
+  if (!ws || ws_bytes < need)
+    return fail(ANUFFT_CAPACITY_EXCEEDED, "workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
+  if (!coords || !values || !out) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  size_t off = 0;
+  for (int i = 0; i < nprob; ++i) {
+    const float *c3[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+    size_t sz = align_up(layout(plans[i], num_points[i]).total);
+    st = anufft_execute_type1(plans[i], num_points[i], c3, stride, values[i], out[i],
+                              (char *)ws + off, sz, stream);
+    if (st != ANUFFT_OK) return st;
+    off += sz;
+  }
+  return ANUFFT_OK;
+}
+
+} // extern "C"
+
+// ---- host-pointer variants ---------------------------------------------------
+namespace {
+
+uint16_t f32_to_f16(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  int32_t exp = (int32_t)((x >> 23) & 0xff) - 127 + 15;
+  uint32_t mant = x & 0x7fffffu;
+  if (((x >> 23) & 0xff) == 0xff) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0u));
+  if (exp >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (exp <= 0) {
+    if (exp < -10) return (uint16_t)sign;
+    mant |= 0x800000u;
+    uint32_t shift = (uint32_t)(14 - exp);
+    uint32_t half = mant >> shift;
+    uint32_t rem = mant & ((1u << shift) - 1u);
+    uint32_t mid = 1u << (shift - 1);
+    if (rem > mid || (rem == mid && (half & 1u))) ++half;
+    return (uint16_t)(sign | half);
+  }
+  uint32_t half = sign | ((uint32_t)exp << 10) | (mant >> 13);
+  uint32_t rem = mant & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (half & 1u))) ++half;
+  return (uint16_t)half;
+}
+
+float f16_to_f32(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  uint32_t exp = (h >> 10) & 0x1f, mant = h & 0x3ffu, x;
+  if (exp == 0) {
+    if (mant == 0) x = sign;
+    else {
+      int e = -1;
+      do {
+        ++e;
+        mant <<= 1;
+      } while (!(mant & 0x400u));
+      x = sign | ((uint32_t)(127 - 15 - e) << 23) | ((mant & 0x3ffu) << 13);
+    }
+  } else if (exp == 31) {
+    x = sign | 0x7f800000u | (mant << 13);
+  } else {
+    x = sign | ((exp + 127 - 15) << 23) | (mant << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+void to_c64(int storage, const void *src, int64_t n, std::vector<float2> &dst) {
+  dst.resize((size_t)std::max<int64_t>(n, 1));
+  for (int64_t i = 0; i < n; ++i) {
+    if (storage == ANUFFT_STORAGE_C128) {
+      const double *s = (const double *)src;
+      dst[i] = make_float2((float)s[2 * i], (float)s[2 * i + 1]);
+    } else if (storage == ANUFFT_STORAGE_C64) {
+      const float *s = (const float *)src;
+      dst[i] = make_float2(s[2 * i], s[2 * i + 1]);
+    } else {
+      const uint16_t *s = (const uint16_t *)src;
+      dst[i] = make_float2(f16_to_f32(s[2 * i]), f16_to_f32(s[2 * i + 1]));
+    }
+  }
+}
+
+void from_c64(int storage, const std::vector<float2> &src, int64_t n, void *dst) {
+  for (int64_t i = 0; i < n; ++i) {
+    if (storage == ANUFFT_STORAGE_C128) {
+      double *d = (double *)dst;
+      d[2 * i] = src[i].x;
+      d[2 * i + 1] = src[i].y;
+    } else if (storage == ANUFFT_STORAGE_C64) {
+      float *d = (float *)dst;
+      d[2 * i] = src[i].x;
+      d[2 * i + 1] = src[i].y;
+    } else {
+      uint16_t *d = (uint16_t *)dst;
+      d[2 * i] = f32_to_f16(src[i].x);
+      d[2 * i + 1] = f32_to_f16(src[i].y);
+    }
+  }
+}
+
+anufft_status host_exec(const anufft_plan *p, int type, int32_t storage, int32_t profile,
+                        int64_t M, const double *const coords[3], int64_t stride,
+                        const void *in, void *out) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  if (storage < 0 || storage > 2) return fail(ANUFFT_INVALID_ARG, "unknown storage %d", storage);
+  if (profile != storage)
+    return fail(ANUFFT_PROFILE_MISMATCH,
+                "invalid field precision_profile: profile %d does not match storage %d", profile,
+                storage);
+  if (M < 0) return fail(ANUFFT_INVALID_ARG, "num_points must be >= 0");
+  if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
+    return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
+  if ((M > 0 || type == 1) && (!in || !out)) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
+  if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  DeviceGuard guard(p->device);
+  hipStream_t s = p->host_stream;
+  const int64_t N = p->modes();
+  const int64_t nin = type == 1 ? M : N, nout = type == 1 ? N : M;
+  std::vector<float2> hin, hout((size_t)std::max<int64_t>(nout, 1));
+  to_c64(storage, in, nin, hin);
+  std::vector<float> hc((size_t)std::max<int64_t>(M, 1) * p->dim);
+  for (int a = 0; a < p->dim; ++a)
+    for (int64_t j = 0; j < M; ++j) hc[(size_t)a * M + j] = (float)coords[a][j * stride];
+  size_t wsb = layout(p, M).total;
+  char *buf = nullptr;
+  size_t b_in = align_up(8 * (size_t)std::max<int64_t>(nin, 1));
+  size_t b_out = align_up(8 * (size_t)std::max<int64_t>(nout, 1));
+  size_t b_c = align_up(4 * hc.size());
+  hipError_t e = hipMalloc(&buf, b_in + b_out + b_c + wsb);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  float2 *d_in = (float2 *)buf, *d_out = (float2 *)(buf + b_in);
+  float *d_c = (float *)(buf + b_in + b_out);
+  void *d_ws = buf + b_in + b_out + b_c;
+  anufft_status st = ANUFFT_OK;
+  do {
+    if ((e = hipMemcpyAsync(d_in, hin.data(), 8 * (size_t)nin, hipMemcpyHostToDevice, s)) != hipSuccess) break;
+    if ((e = hipMemcpyAsync(d_c, hc.data(), 4 * hc.size(), hipMemcpyHostToDevice, s)) != hipSuccess) break;
+    const float *dc[3] = {d_c, p->dim == 3 ? d_c + M : d_c, p->dim == 3 ? d_c + 2 * M : d_c};
+    st = type == 1 ? anufft_execute_type1(p, M, dc, 1, (const float *)d_in, (float *)d_out, d_ws, wsb, s)
+                   : anufft_execute_type2(p, M, dc, 1, (const float *)d_in, (float *)d_out, d_ws, wsb, s);
+    if (st != ANUFFT_OK) break;
+    if ((e = hipMemcpyAsync(hout.data(), d_out, 8 * (size_t)nout, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+    e = hipStreamSynchronize(s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (st != ANUFFT_OK) return st;
+  if (e != hipSuccess) return hip_fail(e, "host execute");
+  from_c64(storage, hout, nout, out);
+  return ANUFFT_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+anufft_status anufft_execute_type1_host(const anufft_plan *p, int32_t storage, int32_t profile,
+                                        int64_t M, const double *const coords[3], int64_t stride,
+                                        const void *values, void *out) {
+  return host_exec(p, 1, storage, profile, M, coords, stride, values, out);
+}
+
+anufft_status anufft_execute_type2_host(const anufft_plan *p, int32_t storage, int32_t profile,
+                                        int64_t M, const double *const coords[3], int64_t stride,
+                                        const void *modes, void *out) {
+  return host_exec(p, 2, storage, profile, M, coords, stride, modes, out);
+}
+
+} // extern "C"

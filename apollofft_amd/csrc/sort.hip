@@ -1,0 +1,278 @@
+// sort.hip -- point binning by a stable LSD counting sort on integer tile keys.
+//
+// Replaces the comparison sorts sort_points_{1d,3d} / sort_positions_{1d,3d}
+// of apollo-nufft (transform/dimension_1d.rs:54-99, dimension_3d.rs:74-161):
+// the reference orders points by bucket with float tie-breaks; here the key
+// is the spreading tile of the point and ties keep the input order, so the
+// permutation is a deterministic function of the coordinates that a CPU
+// replay reproduces bit for bit.
+//
+// Pipeline (one stream, no host synchronisation):
+//   keys   : key[j] = tile of point j, per-chunk histogram of digit 0
+//   per digit pass: scan(counts) -> stable scatter (-> histogram of next digit)
+//   bounds : bin_start[b] = lower_bound(sorted keys, b)
+//   subs   : subproblems of at most max_sub points per tile
+#include "internal.h"
+
+namespace anufft {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ uint32_t tile_key(const PointArgs &pa, const Geom &g, int dim,
+                                             int64_t j) {
+  int cx, cy = 0, cz = 0;
+  fold_coord(pa.x[j * pa.stride], g.scale[0], g.m[0], cx);
+  if (dim == 3) {
+    fold_coord(pa.y[j * pa.stride], g.scale[1], g.m[1], cy);
+    fold_coord(pa.z[j * pa.stride], g.scale[2], g.m[2], cz);
+  }
+  uint32_t bx = (uint32_t)(cx / g.bin[0]);
+  uint32_t by = (uint32_t)(cy / g.bin[1]);
+  uint32_t bz = (uint32_t)(cz / g.bin[2]);
+  return (bx * (uint32_t)g.nbin[1] + by) * (uint32_t)g.nbin[2] + bz;
+}
+
+// keys + digit-0 histogram of each chunk, written digit-major.
+__global__ __launch_bounds__(kSortThreads) void keys_kernel(PointArgs pa, Geom g, int dim,
+                                                            uint32_t *keys, uint32_t *cnt,
+                                                            int nchunks) {
+  __shared__ uint32_t hist[kRadix];
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kSortChunk;
+  for (int i = threadIdx.x; i < kSortChunk; i += kSortThreads) {
+    int64_t j = base + i;
+    if (j < pa.num_points) {
+      uint32_t k = tile_key(pa, g, dim, j);
+      keys[j] = k;
+      atomicAdd(&hist[k & (kRadix - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads)
+    cnt[(int64_t)d * nchunks + blockIdx.x] = hist[d];
+}
+
+// Per-chunk histogram of digit `shift` (passes after the first).
+__global__ __launch_bounds__(kSortThreads) void hist_kernel(const uint32_t *keys, int64_t n,
+                                                            int shift, uint32_t *cnt,
+                                                            int nchunks) {
+  __shared__ uint32_t hist[kRadix];
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d] = 0;
+  __syncthreads();
+  int64_t base = (int64_t)blockIdx.x * kSortChunk;
+  for (int i = threadIdx.x; i < kSortChunk; i += kSortThreads) {
+    int64_t j = base + i;
+    if (j < n) atomicAdd(&hist[(keys[j] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRadix; d += kSortThreads)
+    cnt[(int64_t)d * nchunks + blockIdx.x] = hist[d];
+}
+
+// ---- exclusive scan of a uint32 array (reduce, scan block sums, apply) ----
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
+                                                         uint32_t &total) {
+  // Hillis-Steele over the block in LDS; kScanThreads entries.
+  int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    uint32_t add = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  uint32_t incl = sh[t];
+  total = sh[kScanThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *data,
+                                                                   int64_t n, uint32_t *sums) {
+  __shared__ uint32_t sh[kScanThreads];
+  int64_t base = (int64_t)blockIdx.x * kScanTile;
+  uint32_t s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
+    if (j < n) s += data[j];
+  }
+  uint32_t total;
+  block_exclusive_scan(s, sh, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the block sums, any length.
+__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(uint32_t *sums, int64_t n) {
+  __shared__ uint32_t sh[kScanThreads];
+  uint32_t carry = 0;
+  for (int64_t base = 0; base < n; base += kScanThreads) {
+    int64_t j = base + threadIdx.x;
+    uint32_t v = j < n ? sums[j] : 0u;
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan(v, sh, total);
+    if (j < n) sums[j] = ex + carry;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *data, int64_t n,
+                                                                  const uint32_t *sums) {
+  __shared__ uint32_t sh[kScanThreads];
+  int64_t base = (int64_t)blockIdx.x * kScanTile;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
+    v[i] = j < n ? data[j] : 0u;
+    s += v[i];
+  }
+  uint32_t total;
+  uint32_t ex = block_exclusive_scan(s, sh, total) + sums[blockIdx.x];
+  for (int i = 0; i < kScanItems; ++i) {
+    int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
+    if (j < n) data[j] = ex;
+    ex += v[i];
+  }
+}
+
+hipError_t scan_exclusive(uint32_t *data, int64_t n, uint32_t *tmp, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t nb = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
+                     tmp);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanThreads), 0, s, tmp, nb);
+  hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
+                     tmp);
+  return hipGetLastError();
+}
+
+// ---- stable scatter of one digit --------------------------------------------
+// One block per chunk; the chunk is walked in input order, kSortThreads points
+// per step.  Ranks inside a wave come from matching digits with per-bit
+// ballots; ranks across the waves of a step from per-wave digit counts.
+__global__ __launch_bounds__(kSortThreads) void scatter_kernel(
+    const uint32_t *keys_in, const uint32_t *idx_in, uint32_t *keys_out, uint32_t *idx_out,
+    int64_t n, int shift, const uint32_t *offs, int nchunks) {
+  constexpr int kWaves = kSortThreads / 64;
+  __shared__ uint32_t running[kRadix];
+  __shared__ uint32_t wcount[kWaves][kRadix];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int d = tid; d < kRadix; d += kSortThreads) {
+    running[d] = offs[(int64_t)d * nchunks + blockIdx.x];
+    for (int w = 0; w < kWaves; ++w) wcount[w][d] = 0;
+  }
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int64_t base = (int64_t)blockIdx.x * kSortChunk;
+  for (int step = 0; step < kSortItems; ++step) {
+    int64_t j = base + (int64_t)step * kSortThreads + tid;
+    bool valid = j < n;
+    uint32_t key = valid ? keys_in[j] : 0u;
+    uint32_t idx = valid ? (idx_in ? idx_in[j] : (uint32_t)j) : 0u;
+    uint32_t d = (key >> shift) & (kRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t m = __ballot(valid && bit);
+      peers &= bit ? m : ~m;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && (peers & lt) == 0ull) wcount[wave][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank;
+      for (int w = 0; w < wave; ++w) pos += wcount[w][d];
+      keys_out[pos] = key;
+      idx_out[pos] = idx;
+    }
+    __syncthreads();
+    for (int dd = tid; dd < kRadix; dd += kSortThreads) {
+      uint32_t s = 0;
+      for (int w = 0; w < kWaves; ++w) {
+        s += wcount[w][dd];
+        wcount[w][dd] = 0;
+      }
+      running[dd] += s;
+    }
+    __syncthreads();
+  }
+}
+
+// bin_start[b] = first sorted position with key >= b, b in [0, nbins].
+__global__ void bounds_kernel(const uint32_t *keys, int64_t n, uint32_t *bin_start, int nbins) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < (uint32_t)b) lo = mid + 1;
+    else hi = mid;
+  }
+  bin_start[b] = (uint32_t)lo;
+}
+
+__global__ void subcount_kernel(const uint32_t *bin_start, int nbins, int max_sub,
+                                uint32_t *sub_ofs) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  if (b == nbins) {
+    sub_ofs[b] = 0;
+    return;
+  }
+  uint32_t c = bin_start[b + 1] - bin_start[b];
+  sub_ofs[b] = (c + (uint32_t)max_sub - 1) / (uint32_t)max_sub;
+}
+
+__global__ void subwrite_kernel(const uint32_t *bin_start, const uint32_t *sub_ofs, int nbins,
+                                int max_sub, int4 *subs) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  uint32_t start = bin_start[b], c = bin_start[b + 1] - start;
+  uint32_t o = sub_ofs[b];
+  for (uint32_t s = 0; s * (uint32_t)max_sub < c; ++s) {
+    uint32_t cnt = c - s * (uint32_t)max_sub;
+    if (cnt > (uint32_t)max_sub) cnt = (uint32_t)max_sub;
+    subs[o + s] = make_int4(b, (int)(start + s * (uint32_t)max_sub), (int)cnt, 0);
+  }
+}
+
+} // namespace
+
+hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
+                       hipStream_t s) {
+  const int64_t n = pa.num_points;
+  if (n > 0) {
+    hipLaunchKernelGGL(keys_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s, pa, g, dim,
+                       sb.keys[0], sb.cnt, sb.nchunks);
+    for (int p = 0; p < sb.passes; ++p) {
+      if (p > 0)
+        hipLaunchKernelGGL(hist_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
+                           sb.keys[p & 1], n, p * kRadixBits, sb.cnt, sb.nchunks);
+      hipError_t e = scan_exclusive(sb.cnt, (int64_t)kRadix * sb.nchunks, sb.scan_tmp, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(scatter_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
+                         sb.keys[p & 1], p == 0 ? (const uint32_t *)nullptr : sb.idx[p & 1],
+                         sb.keys[(p + 1) & 1], sb.idx[(p + 1) & 1], n, p * kRadixBits, sb.cnt,
+                         sb.nchunks);
+    }
+  }
+  const uint32_t *sorted = sb.keys[sb.passes & 1];
+  int nb1 = sb.nbins + 1;
+  hipLaunchKernelGGL(bounds_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sorted, n,
+                     sb.bin_start, sb.nbins);
+  hipLaunchKernelGGL(subcount_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sb.bin_start,
+                     sb.nbins, sb.max_sub, sb.sub_ofs);
+  hipError_t e = scan_exclusive(sb.sub_ofs, nb1, sb.scan_tmp, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(subwrite_kernel, dim3((sb.nbins + 255) / 256), dim3(256), 0, s,
+                     sb.bin_start, sb.sub_ofs, sb.nbins, sb.max_sub, sb.subs);
+  return hipGetLastError();
+}
+
+} // namespace anufft

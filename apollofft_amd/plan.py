@@ -1,0 +1,295 @@
+"""Device NUFFT plans over the C ABI (torch tensors for device memory/streams).
+
+``NufftPlan1D`` / ``NufftPlan3D`` mirror apollo-nufft's reusable plans
+(crates/apollo-nufft/src/application/execution/transform/dimension_1d.rs:102-276,
+dimension_3d.rs:163-602): construction from the uniform domain, oversampling
+sigma and KB half-width, then ``type1`` / ``type2`` executions.  Inputs and
+outputs are torch tensors on the plan's GPU (complex64, float32 coordinates);
+work is queued on torch's current stream.  The workspace is a torch uint8
+tensor owned by the plan and grown on demand.
+"""
+import ctypes
+import math
+
+from . import _native as N
+
+
+def _meta_dict(m):
+    return {
+        "dim": m.dim,
+        "n": tuple(m.n),
+        "m": tuple(m.m),
+        "length": tuple(m.length),
+        "kernel_width": m.kernel_width,
+        "sigma": m.sigma,
+        "beta": m.beta,
+        "i0_beta": m.i0_beta,
+        "grid_policy": m.grid_policy,
+        "compat_reference_type2_3d_scaling": bool(m.compat_reference_type2_3d_scaling),
+        "bin_size": tuple(m.bin_size),
+        "max_subproblem": m.max_subproblem,
+        "poly_degree": m.poly_degree,
+        "poly_max_err": m.poly_max_err,
+    }
+
+
+def _policy(p):
+    if p in (None, "reference", N.GRID_REFERENCE):
+        return N.GRID_REFERENCE
+    if p in ("exact", "smooth", N.GRID_EXACT):
+        return N.GRID_EXACT
+    raise ValueError(f"unknown grid_policy {p!r}")
+
+
+class _PlanBase:
+    dim = 0
+
+    def __init__(self):
+        self._h = None
+        self._ws = None
+        self._ws_cap = -1
+
+    def _opts(self, grid_policy, compat, device_id, bin_size, max_subproblem):
+        o = N.default_opts()
+        o.grid_policy = _policy(grid_policy)
+        o.compat_reference_type2_3d_scaling = int(bool(compat))
+        o.device_id = -1 if device_id is None else int(device_id)
+        if bin_size is not None:
+            b = (bin_size,) * 3 if isinstance(bin_size, int) else tuple(bin_size)
+            for a in range(3):
+                o.bin_size[a] = int(b[a])
+        if max_subproblem:
+            o.max_subproblem = int(max_subproblem)
+        return o
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def meta(self):
+        m = N.Meta()
+        N.check(N.lib().anufft_plan_get_meta(self._h, ctypes.byref(m)), "get_meta")
+        return _meta_dict(m)
+
+    def deconv(self, axis=0):
+        """FP64 deconvolution factors of one axis (kaiser_bessel.rs:137-148)."""
+        import numpy as np
+        n = self.meta["n"][axis]
+        out = np.empty(n, dtype=np.float64)
+        N.check(N.lib().anufft_plan_get_deconv(self._h, axis, out.ctypes.data), "get_deconv")
+        return out
+
+    def workspace_size(self, num_points):
+        b = ctypes.c_size_t()
+        N.check(N.lib().anufft_workspace_size(self._h, int(num_points), ctypes.byref(b)),
+                "workspace_size")
+        return b.value
+
+    def workspace(self, num_points, device):
+        import torch
+        need = self.workspace_size(num_points)
+        if self._ws is None or self._ws.numel() < need or self._ws.device != device:
+            self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=device)
+        return self._ws
+
+    def close(self):
+        if self._h:
+            N.lib().anufft_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device execution ------------------------------------------------------
+    def _coords(self, positions):
+        """(ptr_x, ptr_y, ptr_z, stride, M, device) for (M, dim) or tuple inputs."""
+        import torch
+        if isinstance(positions, (tuple, list)):
+            cs = list(positions)
+            if len(cs) != self.dim:
+                raise ValueError(f"expected {self.dim} coordinate arrays")
+            for c in cs:
+                if c.dtype != torch.float32 or not c.is_contiguous() or c.dim() != 1:
+                    raise ValueError("coordinate arrays must be contiguous 1-D float32 tensors")
+            M = cs[0].numel()
+            if any(c.numel() != M for c in cs):
+                raise ValueError("coordinate arrays differ in length")
+            ptrs = [c.data_ptr() for c in cs] + [cs[0].data_ptr()] * (3 - len(cs))
+            return ptrs, 1, M, cs[0].device
+        t = positions
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError("positions must be a contiguous float32 tensor")
+        if self.dim == 1:
+            if t.dim() != 1:
+                raise ValueError("1D positions must have shape (M,)")
+            return [t.data_ptr()] * 3, 1, t.numel(), t.device
+        if t.dim() != 2 or t.shape[1] != 3:
+            raise ValueError("3D positions must have shape (M, 3)")
+        b = t.data_ptr()
+        return [b, b + 4, b + 8], 3, t.shape[0], t.device
+
+    @staticmethod
+    def _stream(device):
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+    def type1(self, positions, values, out=None):
+        """Type-1: non-uniform samples -> modes (dimension_3d.rs:256-330)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        if values.dtype != torch.complex64 or not values.is_contiguous() or values.numel() != M:
+            raise ValueError("values must be a contiguous complex64 tensor with one entry per point")
+        shape = self.modes_shape
+        if out is None:
+            out = torch.empty(shape, dtype=torch.complex64, device=dev)
+        elif tuple(out.shape) != shape or out.dtype != torch.complex64 or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous complex64 tensor of shape {shape}")
+        ws = self.workspace(M, dev)
+        N.check(N.lib().anufft_execute_type1(
+            self._h, M, N.ptr_array(ptrs), stride, values.data_ptr(), out.data_ptr(),
+            ws.data_ptr(), ws.numel(), self._stream(dev)), "execute_type1")
+        return out
+
+    def type2(self, positions, modes, out=None):
+        """Type-2: modes -> non-uniform samples (dimension_3d.rs:496-602)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        shape = self.modes_shape
+        if modes.dtype != torch.complex64 or not modes.is_contiguous() or tuple(modes.shape) != shape:
+            raise ValueError(f"modes must be a contiguous complex64 tensor of shape {shape}")
+        if out is None:
+            out = torch.empty(M, dtype=torch.complex64, device=dev)
+        ws = self.workspace(M, dev)
+        N.check(N.lib().anufft_execute_type2(
+            self._h, M, N.ptr_array(ptrs), stride, modes.data_ptr(), out.data_ptr(),
+            ws.data_ptr(), ws.numel(), self._stream(dev)), "execute_type2")
+        return out
+
+    def grad_points(self, positions, u, modes_h):
+        """grad[a][j] = Re(conj(u_j) d/dx_j^(a) sum_k h_k e^{+i k.x_j}) -> (M, dim)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        if u.dtype != torch.complex64 or not u.is_contiguous() or u.numel() != M:
+            raise ValueError("u must be a contiguous complex64 tensor with one entry per point")
+        if modes_h.dtype != torch.complex64 or not modes_h.is_contiguous() or tuple(modes_h.shape) != self.modes_shape:
+            raise ValueError("modes_h must be a contiguous complex64 tensor of the mode shape")
+        g = torch.empty((self.dim, M), dtype=torch.float32, device=dev)
+        gp = [g[a].data_ptr() for a in range(self.dim)] + [None] * (3 - self.dim)
+        ws = self.workspace(M, dev)
+        N.check(N.lib().anufft_execute_grad_points(
+            self._h, M, N.ptr_array(ptrs), stride, u.data_ptr(), modes_h.data_ptr(),
+            N.ptr_array(gp, 3), ws.data_ptr(), ws.numel(), self._stream(dev)), "grad_points")
+        return g.t()
+
+    # ---- host drop-in execution (numpy; copies in and out) -------------------------
+    def _host_coords(self, positions):
+        import numpy as np
+        pos = np.asarray(positions, dtype=np.float64)
+        if self.dim == 1:
+            pos = np.ascontiguousarray(pos.reshape(-1))
+            M = pos.size
+            base = pos.ctypes.data
+            return pos, [base] * 3, 1, M
+        pos = np.ascontiguousarray(pos.reshape(-1, 3))
+        base = pos.ctypes.data
+        return pos, [base, base + 8, base + 16], 3, pos.shape[0]
+
+    def type1_host(self, positions, values, storage=N.STORAGE_C128, profile=None):
+        import numpy as np
+        keep, ptrs, stride, M = self._host_coords(positions)
+        dt = {N.STORAGE_C128: np.complex128, N.STORAGE_C64: np.complex64, N.STORAGE_F16X2: np.float16}[storage]
+        vals = np.ascontiguousarray(values, dtype=dt)
+        if storage == N.STORAGE_F16X2 and vals.shape[-1:] != (2,):
+            raise ValueError("f16 storage takes (..., 2) half arrays")
+        nvals = vals.size // 2 if storage == N.STORAGE_F16X2 else vals.size
+        if nvals != M:
+            raise N.ShapeMismatch(f"positions/value length mismatch: {M} vs {nvals}")
+        shape = self.modes_shape + ((2,) if storage == N.STORAGE_F16X2 else ())
+        out = np.empty(shape, dtype=dt)
+        N.check(N.lib().anufft_execute_type1_host(
+            self._h, storage, storage if profile is None else profile, M, N.ptr_array(ptrs),
+            stride, vals.ctypes.data, out.ctypes.data), "execute_type1_host")
+        return out
+
+    def type2_host(self, positions, modes, storage=N.STORAGE_C128, profile=None):
+        import numpy as np
+        keep, ptrs, stride, M = self._host_coords(positions)
+        dt = {N.STORAGE_C128: np.complex128, N.STORAGE_C64: np.complex64, N.STORAGE_F16X2: np.float16}[storage]
+        md = np.ascontiguousarray(modes, dtype=dt)
+        nmodes = md.size // 2 if storage == N.STORAGE_F16X2 else md.size
+        if nmodes != math.prod(self.modes_shape):
+            raise N.ShapeMismatch(f"modes shape mismatch: expected {self.modes_shape}")
+        out = np.empty((M, 2) if storage == N.STORAGE_F16X2 else M, dtype=dt)
+        N.check(N.lib().anufft_execute_type2_host(
+            self._h, storage, storage if profile is None else profile, M, N.ptr_array(ptrs),
+            stride, md.ctypes.data, out.ctypes.data), "execute_type2_host")
+        return out
+
+
+class NufftPlan1D(_PlanBase):
+    """1D plan: NufftPlan1D::new(UniformDomain1D(n, dx), sigma, kernel_width)
+    (dimension_1d.rs:127-154)."""
+
+    dim = 1
+
+    def __init__(self, n, dx, sigma=2, kernel_width=6, grid_policy="reference",
+                 device_id=None):
+        super().__init__()
+        h = ctypes.c_void_p()
+        o = self._opts(grid_policy, False, device_id, None, None)
+        N.check(N.lib().anufft_plan1d_create(int(n), float(dx), int(sigma), int(kernel_width),
+                                             ctypes.byref(o), ctypes.byref(h)), "plan1d_create")
+        self._h = h
+        self.modes_shape = (int(n),)
+
+
+class NufftPlan3D(_PlanBase):
+    """3D plan: NufftPlan3D::new(UniformGrid3D(nx,ny,nz,dx,dy,dz), sigma, w)
+    (dimension_3d.rs:193-232)."""
+
+    dim = 3
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, sigma=2, kernel_width=6,
+                 grid_policy="reference", compat_reference_type2_3d_scaling=False,
+                 device_id=None, bin_size=None, max_subproblem=None):
+        super().__init__()
+        h = ctypes.c_void_p()
+        o = self._opts(grid_policy, compat_reference_type2_3d_scaling, device_id, bin_size,
+                       max_subproblem)
+        N.check(N.lib().anufft_plan3d_create(int(nx), int(ny), int(nz), float(dx), float(dy),
+                                             float(dz), int(sigma), int(kernel_width),
+                                             ctypes.byref(o), ctypes.byref(h)), "plan3d_create")
+        self._h = h
+        self.modes_shape = (int(nx), int(ny), int(nz))
+
+
+def execute_type1_batched(plans, positions, values, outs=None):
+    """Batched Type-1 over independent 3D plans on torch's current stream."""
+    import torch
+    n = len(plans)
+    if not (len(positions) == len(values) == n):
+        raise ValueError("plans, positions and values differ in length")
+    coords, vals, Ms, dev = [], [], [], None
+    for p, pos, v in zip(plans, positions, values):
+        ptrs, stride, M, dev = p._coords(pos)
+        if stride != 3 and n > 0:
+            raise ValueError("batched execution takes (M, 3) position tensors")
+        coords += ptrs
+        vals.append(v.data_ptr())
+        Ms.append(M)
+    if outs is None:
+        outs = [torch.empty(p.modes_shape, dtype=torch.complex64, device=dev) for p in plans]
+    hs = N.ptr_array([p.handle.value for p in plans])
+    Marr = (ctypes.c_int64 * n)(*Ms)
+    b = ctypes.c_size_t()
+    N.check(N.lib().anufft_batch_workspace_size(hs, n, Marr, ctypes.byref(b)), "batch_workspace_size")
+    ws = torch.empty(max(b.value, 256), dtype=torch.uint8, device=dev)
+    N.check(N.lib().anufft_execute_type1_batched(
+        hs, n, Marr, N.ptr_array(coords), 3, N.ptr_array(vals),
+        N.ptr_array([o.data_ptr() for o in outs]), ws.data_ptr(), ws.numel(),
+        _PlanBase._stream(dev)), "execute_type1_batched")
+    return outs

@@ -1,0 +1,231 @@
+/*
+ * apollo_nufft.h -- C ABI of the MI355X (gfx950) Kaiser-Bessel NUFFT library
+ * (libapollo_nufft_hip.so).
+ *
+ * Drop-in boundary for apollo-nufft's plan/execute API.  Each entry point
+ * names the reference interface it replaces (paths relative to the apollofft
+ * repository root).  Conventions kept from the reference:
+ *   - kernel_width is the KB half-width w: 2w+1 taps per axis, 2w non-zero
+ *     (apollo-nufft/src/infrastructure/kernel/kaiser_bessel.rs:109-116);
+ *   - beta = pi (1 - 1/(2 sigma)) 2w with the integer sigma
+ *     (apollo-nufft/src/application/execution/transform/dimension_3d.rs:200);
+ *   - Type-1 f_k = sum_j c_j exp(-2 pi i k.x_j/L), Type-2
+ *     g_j = sum_k f_k exp(+2 pi i k.x_j/L) (dimension_1d.rs:10-19);
+ *   - modes in FFT order with k <= n/2 taken as positive
+ *     (kaiser_bessel.rs:152-158), 3D arrays row-major [kx][ky][kz]
+ *     (dimension_3d.rs:400-413, ndarray Array3);
+ *   - positions may be anywhere on the real line; they are folded into
+ *     [0, L) per axis (dimension_3d.rs:93-95).
+ * Complex data are interleaved float pairs (Complex32 / torch.complex64).
+ *
+ * Ownership: plans own read-only device tables (deconvolution factors in
+ * FP64, FP64-built twiddles, kernel polynomial coefficients).  The caller
+ * owns inputs, outputs and the workspace; execute calls allocate nothing.
+ * Execution is asynchronous on the given hipStream_t (NULL = legacy default
+ * stream).  Plans are immutable after creation and may be shared between
+ * threads; calls that share one workspace must not overlap.
+ * Every call returns a status; anufft_last_error() gives the message of the
+ * last failure on the calling thread.  Nothing aborts.
+ */
+#ifndef APOLLO_NUFFT_H
+#define APOLLO_NUFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Mirrors apollo-fft ApolloError (apollo-fft/src/domain/
+ * contracts/error.rs:10-63), NufftMetadataError (apollo-nufft/src/domain/
+ * metadata/grid.rs:7-23) and NufftWgpuError (apollo-nufft-wgpu/src/domain/
+ * error.rs:10-49). */
+typedef enum anufft_status {
+  ANUFFT_OK = 0,
+  ANUFFT_INVALID_ARG = 1,        /* NufftMetadataError, sigma/w < 2, InvalidPlan */
+  ANUFFT_SHAPE_MISMATCH = 2,     /* ShapeMismatch / InputLengthMismatch */
+  ANUFFT_PROFILE_MISMATCH = 3,   /* Validation{field:"precision_profile"} */
+  ANUFFT_CAPACITY_EXCEEDED = 4,  /* workspace smaller than anufft_workspace_size */
+  ANUFFT_U32_OVERFLOW = 5,       /* validate_usize_to_u32 (device.rs:973-980) */
+  ANUFFT_DEVICE_UNAVAILABLE = 6, /* AdapterUnavailable / DeviceUnavailable */
+  ANUFFT_HIP_ERROR = 7,          /* HIP runtime error */
+  ANUFFT_UNSUPPORTED = 8         /* UnsupportedExecution */
+} anufft_status;
+
+/* Oversampled-grid policy. */
+typedef enum anufft_grid_policy {
+  /* Reference rule: 3D m = next_pow2(max(sigma n, 2w+1)) (dimension_3d.rs:48-54,
+   * device.rs:856-891); 1D m = sigma n (dimension_1d.rs:131). */
+  ANUFFT_GRID_REFERENCE = 0,
+  /* m = max(sigma n, 2w+1) exactly: mixed-radix grids such as 200 or 254. */
+  ANUFFT_GRID_EXACT = 1
+} anufft_grid_policy;
+
+typedef struct anufft_opts {
+  int32_t grid_policy;                       /* anufft_grid_policy */
+  int32_t compat_reference_type2_3d_scaling; /* 1: 3D T2 returns direct/(mx my mz) like the reference */
+  int32_t device_id;                         /* -1: current device */
+  int32_t bin_size[3];                       /* spreading tile in fine cells; 0 = automatic */
+  int32_t max_subproblem;                    /* points per spreading subproblem; 0 = automatic */
+} anufft_opts;
+
+typedef struct anufft_meta {
+  int32_t dim;          /* 1 or 3 */
+  int64_t n[3];         /* modes per axis (1D: n[1] = n[2] = 1) */
+  int64_t m[3];         /* oversampled grid per axis */
+  double length[3];     /* periodic lengths L = n dx */
+  int32_t kernel_width; /* w (half-width) */
+  int32_t sigma;
+  double beta;
+  double i0_beta;
+  int32_t grid_policy;
+  int32_t compat_reference_type2_3d_scaling;
+  int32_t bin_size[3];
+  int32_t max_subproblem;
+  int32_t poly_degree;  /* degree of the per-tap kernel polynomials */
+  double poly_max_err;  /* max |phi - fit| on [-w, w] measured at plan creation */
+} anufft_meta;
+
+typedef struct anufft_plan anufft_plan;
+
+/* Fill opts with defaults (reference grid policy, automatic tiles). */
+void anufft_default_opts(anufft_opts *opts);
+
+/* NufftPlan1D::new (apollo-nufft .../transform/dimension_1d.rs:127-154),
+ * NufftWgpuPlan1D::new (apollo-nufft-wgpu/src/application/plan.rs:16) and
+ * UniformDomain1D::new validation (domain/metadata/grid.rs:36-48). */
+anufft_status anufft_plan1d_create(int64_t n, double dx, int32_t sigma, int32_t kernel_width,
+                                   const anufft_opts *opts, anufft_plan **plan);
+
+/* NufftPlan3D::new (dimension_3d.rs:193-232), NufftWgpuPlan3D::new
+ * (application/plan.rs:54), UniformGrid3D::new validation (grid.rs:76-110). */
+anufft_status anufft_plan3d_create(int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                                   double dz, int32_t sigma, int32_t kernel_width,
+                                   const anufft_opts *opts, anufft_plan **plan);
+
+anufft_status anufft_plan_destroy(anufft_plan *plan);
+
+/* KB metadata of the plan: the private fields m, beta, i0_beta of
+ * NufftPlan{1D,3D} (dimension_1d.rs:102-111, dimension_3d.rs:164-178) and the
+ * GPU copy in fast_{1d,3d}_metadata (apollo-nufft-wgpu .../device.rs:842-1034). */
+anufft_status anufft_plan_get_meta(const anufft_plan *plan, anufft_meta *meta);
+
+/* FP64 deconvolution factors 1/phi_hat(k_signed/m) of one axis, n values
+ * (axis_deconv, kaiser_bessel.rs:137-148). */
+anufft_status anufft_plan_get_deconv(const anufft_plan *plan, int32_t axis, double *out_n);
+
+/* Workspace bytes for executing up to max_points points with this plan
+ * (replaces NufftGpuBuffers{1D,3D}::new capacity, apollo-nufft-wgpu
+ * .../kernel.rs:82-181,234). */
+anufft_status anufft_workspace_size(const anufft_plan *plan, int64_t max_points, size_t *bytes);
+
+/* Type-1, device pointers.  Replaces NufftPlan{1D,3D}::type1_into
+ * (dimension_1d.rs:166, dimension_3d.rs:256) and NufftWgpuBackend::
+ * execute_fast_type1_{1d,3d}[_with_buffers] (device.rs:217,383,681,752).
+ * Point coordinate j of axis a is read at coords[a][j * coord_stride]
+ * (SoA: stride 1; AoS (x,y,z) tuples: stride 3).  1D plans read coords[0]
+ * only.  values: M complex; out: n (1D) or nx*ny*nz (3D) complex. */
+anufft_status anufft_execute_type1(const anufft_plan *plan, int64_t num_points,
+                                   const float *const coords[3], int64_t coord_stride,
+                                   const float *values, float *out, void *workspace,
+                                   size_t workspace_bytes, void *hip_stream);
+
+/* Type-2, device pointers.  Replaces NufftPlan{1D,3D}::type2_into
+ * (dimension_1d.rs:224, dimension_3d.rs:496) and execute_fast_type2_{1d,3d}
+ * (device.rs:277,446).  modes: n or nx*ny*nz complex; out: M complex. */
+anufft_status anufft_execute_type2(const anufft_plan *plan, int64_t num_points,
+                                   const float *const coords[3], int64_t coord_stride,
+                                   const float *modes, float *out, void *workspace,
+                                   size_t workspace_bytes, void *hip_stream);
+
+/* Gradient of a real loss with respect to the point coordinates, for both
+ * transform types (no reference counterpart; the reference has no autograd).
+ * Given per-point complex u_j and modes h, writes for each axis a
+ *   grad[a][j] = Re( conj(u_j) * d/dx_j^(a) [ sum_k h_k exp(+2 pi i k.x_j/L) ] )
+ * using the kernel derivative on the oversampled grid.
+ *   Type-2 forward c = T2(f,x), upstream gradient g on c:  u = g, h = f.
+ *   Type-1 forward F = T1(c,x), upstream gradient G on F:  u = c, h = G.
+ * grad[a] may be NULL to skip an axis (3D) and must be non-NULL for a=0. */
+anufft_status anufft_execute_grad_points(const anufft_plan *plan, int64_t num_points,
+                                         const float *const coords[3], int64_t coord_stride,
+                                         const float *u, const float *modes_h,
+                                         float *const grad[3], void *workspace,
+                                         size_t workspace_bytes, void *hip_stream);
+
+/* Batched Type-1 over independent 3D plans (no reference counterpart: the
+ * reference has no batched NUFFT).  Problem p uses plans[p], num_points[p],
+ * coords_p[a] = coords[3p + a] (stride coord_stride), values[p], out[p].  The
+ * workspace must hold anufft_batch_workspace_size bytes.  All problems run in
+ * one stream; binning and spreading of every problem are fused into shared
+ * launches. */
+anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32_t num_problems,
+                                          const int64_t *max_points, size_t *bytes);
+anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int32_t num_problems,
+                                           const int64_t *num_points,
+                                           const float *const *coords, int64_t coord_stride,
+                                           const float *const *values, float *const *out,
+                                           void *workspace, size_t workspace_bytes,
+                                           void *hip_stream);
+
+/* Host-pointer variants for drop-in parity tests: copy in, execute on an
+ * internal stream of the plan's device, copy out, synchronize.  They allocate
+ * device memory per call (not for hot loops).  Coordinates are FP64 host
+ * arrays (like NufftPlan{1D,3D}::type1_into, dimension_1d.rs:166) rounded
+ * once to FP32 for the device, like the WGPU backend's f32 positions
+ * (device.rs:383).  Values use `storage`; the typed reference paths
+ * (type1_typed_into / type2_typed_into, dimension_1d.rs:283-349,
+ * dimension_3d.rs:333-389,605-629, WGPU device.rs:142-675) require `profile`
+ * to match the storage exactly (validate_profile, dimension_1d.rs:403-419),
+ * else ANUFFT_PROFILE_MISMATCH.  Arithmetic is FP32 in every case (the WGPU
+ * backend's rule, device.rs:248-252); FP16 storage is promoted once on input
+ * and quantized once on output (dimension_1d.rs:388-401). */
+typedef enum anufft_storage {
+  ANUFFT_STORAGE_C128 = 0,  /* Complex64 (f64 pairs)  <-> HIGH_ACCURACY_F64 */
+  ANUFFT_STORAGE_C64 = 1,   /* Complex32 (f32 pairs)  <-> LOW_PRECISION_F32 */
+  ANUFFT_STORAGE_F16X2 = 2  /* [f16; 2] (IEEE half)   <-> MIXED_PRECISION_F16_F32 */
+} anufft_storage;
+
+typedef enum anufft_profile {
+  ANUFFT_PROFILE_HIGH_ACCURACY_F64 = 0,
+  ANUFFT_PROFILE_LOW_PRECISION_F32 = 1,
+  ANUFFT_PROFILE_MIXED_PRECISION_F16_F32 = 2
+} anufft_profile;
+
+anufft_status anufft_execute_type1_host(const anufft_plan *plan, int32_t storage,
+                                        int32_t profile, int64_t num_points,
+                                        const double *const coords[3], int64_t coord_stride,
+                                        const void *values, void *out);
+anufft_status anufft_execute_type2_host(const anufft_plan *plan, int32_t storage,
+                                        int32_t profile, int64_t num_points,
+                                        const double *const coords[3], int64_t coord_stride,
+                                        const void *modes, void *out);
+
+/* Diagnostics for stage-level parity tests (no reference counterpart).
+ * anufft_debug_sort: the point binning of a 3D plan -- perm[i] is the input
+ * index of the i-th point in tile order (stable: ties keep input order),
+ * bin_start[b] (nbins + 1 entries) the first sorted position of tile b.
+ * anufft_debug_spread: the oversampled grid after Type-1 spreading
+ * (mx*my*mz complex, before the FFT), the quantity apollo-nufft accumulates
+ * in type1_into (dimension_3d.rs:277-327). */
+anufft_status anufft_debug_num_bins(const anufft_plan *plan, int64_t *num_bins);
+anufft_status anufft_debug_sort(const anufft_plan *plan, int64_t num_points,
+                                const float *const coords[3], int64_t coord_stride,
+                                uint32_t *perm, uint32_t *bin_start, void *workspace,
+                                size_t workspace_bytes, void *hip_stream);
+anufft_status anufft_debug_spread(const anufft_plan *plan, int64_t num_points,
+                                  const float *const coords[3], int64_t coord_stride,
+                                  const float *values, float *grid, void *workspace,
+                                  size_t workspace_bytes, void *hip_stream);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *anufft_last_error(void);
+
+/* Library version string. */
+const char *anufft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APOLLO_NUFFT_H */

@@ -1,0 +1,281 @@
+"""Device parity tests (HIP path through the C ABI) against the FP64 oracle.
+
+Every test runs the HIP kernels of libapollo_nufft_hip.so on cuda:0 and
+compares with the oracle (FP64 direct sums, the FP64 restatement of
+apollo-nufft, or the numpy replay of the binning) on the same float32
+coordinates.  Tolerance for floating point: relative L2 error <= 1e-5
+against the FP64 direct sum (the target accuracy of the workload), and the
+binning permutation must match the CPU replay exactly.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import TWO_PI, clustered_points, cnormal, max_rel, rel_l2, uniform_points
+from replay import replay_sort
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _t(x, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _plan3(n, w=6, policy="reference", compat=False, bins=None, max_sub=None):
+    import apollofft_amd as A
+    d = [TWO_PI / k for k in n]
+    return A.NufftPlan3D(n[0], n[1], n[2], d[0], d[1], d[2], 2, w, grid_policy=policy,
+                         compat_reference_type2_3d_scaling=compat, bin_size=bins,
+                         max_subproblem=max_sub), d
+
+
+# ---------------------------------------------------------------- binning
+@pytest.mark.parametrize("case", ["uniform", "clustered", "edges"])
+def test_binning_matches_cpu_replay(dev, case):
+    import apollofft_amd._native as N
+    import torch
+    rng = np.random.default_rng(11)
+    n = (24, 20, 16)
+    plan, d = _plan3(n, w=5, policy="exact")
+    meta = plan.meta
+    if case == "uniform":
+        xyz = uniform_points(rng, 50000)
+    elif case == "clustered":
+        xyz = clustered_points(rng, 50000)
+    else:  # boundary values, exact multiples of L, tiny negatives, far periods
+        vals = np.array([-np.pi, np.pi, 0.0, -1e-30, 1e-30, -1e-7, TWO_PI, -TWO_PI,
+                         3 * TWO_PI + 0.1, -5 * TWO_PI - 0.2, np.nextafter(np.float32(np.pi), 0)],
+                        dtype=np.float32)
+        xyz = rng.choice(vals, size=(3000, 3)).astype(np.float32)
+    M = xyz.shape[0]
+    keys, perm_ref, bs_ref = replay_sort(xyz, n, d, meta["m"], meta["bin_size"])
+    nb = ctypes.c_int64()
+    N.check(N.lib().anufft_debug_num_bins(plan.handle, ctypes.byref(nb)))
+    assert nb.value + 1 == bs_ref.size
+    txyz = _t(xyz, dev)
+    perm = torch.empty(M, dtype=torch.int32, device=dev)
+    bs = torch.empty(nb.value + 1, dtype=torch.int32, device=dev)
+    ws = plan.workspace(M, dev)
+    b = txyz.data_ptr()
+    N.check(N.lib().anufft_debug_sort(plan.handle, M, N.ptr_array([b, b + 4, b + 8]), 3,
+                                      perm.data_ptr(), bs.data_ptr(), ws.data_ptr(), ws.numel(),
+                                      None))
+    torch.cuda.synchronize()
+    assert np.array_equal(perm.cpu().numpy().view(np.uint32), perm_ref)
+    assert np.array_equal(bs.cpu().numpy().view(np.uint32), bs_ref)
+
+
+# ---------------------------------------------------------------- spreading
+@pytest.mark.parametrize("w", [2, 4, 6, 7, 8])
+def test_spread_fine_grid_matches_oracle(dev, oracle_mod, w):
+    import apollofft_amd._native as N
+    import torch
+    rng = np.random.default_rng(w)
+    n = (12, 10, 16)
+    plan, d = _plan3(n, w=w)
+    meta = plan.meta
+    xyz = uniform_points(rng, 3000)
+    c = cnormal(rng, 3000)
+    ref = oracle_mod.spread3d(xyz.astype(np.float64), c.astype(np.complex128), n, d, 2, w,
+                              oracle_mod.POLICY_POW2)
+    grid = torch.empty(tuple(meta["m"]), dtype=torch.complex64, device=dev)
+    txyz, tc = _t(xyz, dev), _t(c, dev)
+    ws = plan.workspace(3000, dev)
+    b = txyz.data_ptr()
+    N.check(N.lib().anufft_debug_spread(plan.handle, 3000, N.ptr_array([b, b + 4, b + 8]), 3,
+                                        tc.data_ptr(), grid.data_ptr(), ws.data_ptr(), ws.numel(),
+                                        None))
+    g = grid.cpu().numpy()
+    assert rel_l2(g, ref) < 1e-6
+
+
+# ---------------------------------------------------------------- 3D type 1 / 2
+CASES_3D = [
+    ((16, 16, 16), 6, "reference"),
+    ((12, 10, 16), 7, "reference"),
+    ((15, 9, 10), 4, "exact"),    # m = 30, 18, 20 (radix 2,3,5)
+    ((13, 7, 11), 8, "exact"),    # m = 26, 17 (max(14,17)), 22: prime radices 13, 17, 11
+    ((8, 8, 8), 2, "reference"),
+    ((20, 24, 18), 5, "exact"),
+]
+
+
+@pytest.mark.parametrize("n,w,policy", CASES_3D)
+def test_type1_3d_vs_direct(dev, oracle_mod, n, w, policy):
+    rng = np.random.default_rng(100 + w)
+    plan, d = _plan3(n, w=w, policy=policy)
+    M = 4000
+    xyz = uniform_points(rng, M)
+    c = cnormal(rng, M)
+    out = plan.type1(_t(xyz, dev), _t(c, dev)).cpu().numpy()
+    ref = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d)
+    fast = oracle_mod.nufft3d_type1_fast(xyz.astype(np.float64), c.astype(np.complex128), n, d,
+                                         2, w, 1 if policy == "exact" else 0)
+    kb_err = rel_l2(fast, ref)  # the algorithm's own (FP64) truncation error
+    err = rel_l2(out, ref)
+    assert err < max(TOL, 2.0 * kb_err), (err, kb_err)
+
+
+@pytest.mark.parametrize("n,w,policy", CASES_3D)
+def test_type2_3d_vs_direct(dev, oracle_mod, n, w, policy):
+    rng = np.random.default_rng(200 + w)
+    plan, d = _plan3(n, w=w, policy=policy)
+    M = 3000
+    xyz = uniform_points(rng, M)
+    f = cnormal(rng, n)
+    out = plan.type2(_t(xyz, dev), _t(f, dev)).cpu().numpy()
+    ref = oracle_mod.direct_type2_3d(f.astype(np.complex128), xyz.astype(np.float64), d)
+    fast = oracle_mod.nufft3d_type2_fast(f.astype(np.complex128), xyz.astype(np.float64), d, 2, w,
+                                         1 if policy == "exact" else 0)
+    kb_err = rel_l2(fast, ref)
+    err = rel_l2(out, ref)
+    assert err < max(TOL, 2.0 * kb_err), (err, kb_err)
+
+
+def test_type2_3d_compat_scaling(dev, oracle_mod):
+    """compat flag reproduces the reference's direct/(mx my mz) output
+    (dimension_3d.rs:549-551)."""
+    rng = np.random.default_rng(5)
+    n = (8, 8, 8)
+    plan, d = _plan3(n, w=6, compat=True)
+    xyz = uniform_points(rng, 500)
+    f = cnormal(rng, n)
+    out = plan.type2(_t(xyz, dev), _t(f, dev)).cpu().numpy()
+    ref = oracle_mod.nufft3d_type2_fast(f.astype(np.complex128), xyz.astype(np.float64), d, 2, 6,
+                                        0, compat_scaling=True)
+    assert rel_l2(out, ref) < TOL
+
+
+def test_type1_clustered_and_tiny_subproblems(dev, oracle_mod):
+    """Clustered points force many subproblems per tile (max_subproblem=64)."""
+    rng = np.random.default_rng(7)
+    n = (16, 16, 16)
+    plan, d = _plan3(n, w=6, max_sub=64)
+    xyz = clustered_points(rng, 20000, sigma=0.05)
+    c = cnormal(rng, 20000)
+    out = plan.type1(_t(xyz, dev), _t(c, dev)).cpu().numpy()
+    ref = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d)
+    assert rel_l2(out, ref) < TOL
+
+
+def test_empty_inputs(dev):
+    import torch
+    plan, d = _plan3((8, 8, 8), w=4)
+    out = plan.type1(torch.empty((0, 3), dtype=torch.float32, device=dev),
+                     torch.empty(0, dtype=torch.complex64, device=dev))
+    assert torch.count_nonzero(out).item() == 0
+    f = torch.ones((8, 8, 8), dtype=torch.complex64, device=dev)
+    out2 = plan.type2(torch.empty((0, 3), dtype=torch.float32, device=dev), f)
+    assert out2.numel() == 0
+
+
+def test_soa_and_aos_coordinates_agree(dev):
+    import torch
+    rng = np.random.default_rng(9)
+    plan, d = _plan3((16, 16, 16), w=6)
+    xyz = uniform_points(rng, 5000)
+    c = _t(cnormal(rng, 5000), dev)
+    a = plan.type1(_t(xyz, dev), c)
+    soa = tuple(_t(np.ascontiguousarray(xyz[:, k]), dev) for k in range(3))
+    b = plan.type1(soa, c)
+    torch.cuda.synchronize()
+    assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-6
+
+
+# ---------------------------------------------------------------- 1D
+def test_type1_1d_c1_config(dev, oracle_mod):
+    """1D Type-1: M=4096 uniform in [-pi, pi), N=256, sigma=2, w=6 (m=512)."""
+    import apollofft_amd as A
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-np.pi, np.pi, 4096).astype(np.float32)
+    c = cnormal(rng, 4096)
+    plan = A.NufftPlan1D(256, TWO_PI / 256, 2, 6)
+    assert plan.meta["m"][0] == 512
+    out = plan.type1(_t(x, dev), _t(c, dev)).cpu().numpy()
+    ref = oracle_mod.direct_type1_1d(x.astype(np.float64), c.astype(np.complex128), 256, TWO_PI / 256)
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("n,w", [(32, 6), (33, 4), (8, 6), (4, 6), (100, 8)])
+def test_1d_type1_type2_vs_direct(dev, oracle_mod, n, w):
+    import apollofft_amd as A
+    rng = np.random.default_rng(n)
+    dx = 0.05
+    L = n * dx
+    x = rng.uniform(-L, 2 * L, 700).astype(np.float32)
+    c = cnormal(rng, 700)
+    plan = A.NufftPlan1D(n, dx, 2, w)
+    out = plan.type1(_t(x, dev), _t(c, dev)).cpu().numpy()
+    ref = oracle_mod.direct_type1_1d(x.astype(np.float64), c.astype(np.complex128), n, dx)
+    fast = oracle_mod.nufft1d_type1_fast(x.astype(np.float64), c.astype(np.complex128), n, dx, 2, w)
+    assert rel_l2(out, ref) < max(TOL, 2 * rel_l2(fast, ref))
+    f = cnormal(rng, n)
+    out2 = plan.type2(_t(x, dev), _t(f, dev)).cpu().numpy()
+    ref2 = oracle_mod.direct_type2_1d(f.astype(np.complex128), x.astype(np.float64), dx)
+    fast2 = oracle_mod.nufft1d_type2_fast(f.astype(np.complex128), x.astype(np.float64), dx, 2, w)
+    assert rel_l2(out2, ref2) < max(TOL, 2 * rel_l2(fast2, ref2))
+
+
+# ---------------------------------------------------------------- gradient
+def test_grad_points_type2_vs_direct(dev, oracle_mod):
+    """d/dx of L = ||T2(f,x) - b||^2 against the closed-form direct sum."""
+    import torch
+    rng = np.random.default_rng(3)
+    n = (12, 12, 12)
+    plan, d = _plan3(n, w=7)
+    M = 1500
+    xyz = uniform_points(rng, M)
+    f = cnormal(rng, n)
+    b = cnormal(rng, M)
+    x64 = xyz.astype(np.float64)
+    cvals = oracle_mod.direct_type2_3d(f.astype(np.complex128), x64, d)
+    r = cvals - b.astype(np.complex128)
+    ref = oracle_mod.direct_type2_3d_grad_x(f.astype(np.complex128), x64, d, r)
+    g = (2.0 * r).astype(np.complex64)  # torch's upstream gradient of ||.||^2
+    out = plan.grad_points(_t(xyz, dev), _t(g, dev), _t(f, dev))
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), ref) < TOL
+
+
+# ---------------------------------------------------------------- host drop-in API
+def test_host_api_matches_direct(dev, oracle_mod):
+    import apollofft_amd as A
+    rng = np.random.default_rng(21)
+    pos = rng.uniform(-np.pi, np.pi, (2000, 3))
+    vals = rng.standard_normal(2000) + 1j * rng.standard_normal(2000)
+    n = 16
+    out = A.nufft_type1_3d_fast(pos, vals, n, n, n, TWO_PI / n, TWO_PI / n, TWO_PI / n)
+    ref = oracle_mod.direct_type1_3d(pos.astype(np.float32).astype(np.float64), vals, (n,) * 3,
+                                     (TWO_PI / n,) * 3)
+    assert out.dtype == np.complex128 and out.shape == (n, n, n)
+    assert rel_l2(out, ref) < TOL
+    modes = rng.standard_normal((n, n, n)) + 1j * rng.standard_normal((n, n, n))
+    g = A.nufft_type2_3d_fast(pos, modes, TWO_PI / n, TWO_PI / n, TWO_PI / n)
+    refg = oracle_mod.direct_type2_3d(modes, pos.astype(np.float32).astype(np.float64),
+                                      (TWO_PI / n,) * 3)
+    assert rel_l2(g, refg) < TOL
+
+
+def test_host_profile_mismatch_and_f16(dev):
+    import apollofft_amd as A
+    import apollofft_amd._native as N
+    plan = A.NufftPlan1D(8, 0.125, 2, 6)
+    pos = np.array([0.0, 0.09, 0.21, 0.47])
+    v64 = np.array([1 + 0.25j, -0.5 + 0.75j, 0.25 - 0.1j, 0.125 + 0.5j])
+    with pytest.raises(A.ProfileMismatch):
+        plan.type1_host(pos, v64.astype(np.complex64), storage=N.STORAGE_C64,
+                        profile=N.PROFILE_HIGH_ACCURACY_F64)
+    expected = plan.type1_host(pos, v64)
+    v16 = np.stack([v64.real, v64.imag], axis=-1).astype(np.float16)
+    out16 = plan.type1_host(pos, v16, storage=N.STORAGE_F16X2)
+    e16 = plan.type1_host(pos, v16[..., 0].astype(np.float64) + 1j * v16[..., 1].astype(np.float64))
+    # f16 output is the f32 result quantized once (dimension_1d.rs:388-401)
+    bound_re = np.abs(e16.real) * 2.0 ** -10 + 2.0 ** -14
+    bound_im = np.abs(e16.imag) * 2.0 ** -10 + 2.0 ** -14
+    assert np.all(np.abs(out16[:, 0].astype(np.float64) - e16.real) <= bound_re + 1e-6)
+    assert np.all(np.abs(out16[:, 1].astype(np.float64) - e16.imag) <= bound_im + 1e-6)
+    assert expected.shape == (8,)
