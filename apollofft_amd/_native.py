@@ -41,7 +41,10 @@ EXPORTS = [
     "anufft_execute_type1_batched", "anufft_execute_type1_host",
     "anufft_execute_type2_host", "anufft_last_error", "anufft_version",
     "anufft_debug_num_bins", "anufft_debug_sort", "anufft_debug_spread",
+    "anufft_timing_enable", "anufft_timing_collect",
 ]
+
+STAGES = ["fill", "bin", "spread", "fft", "interp", "grad"]
 
 
 class NufftError(RuntimeError):
@@ -158,6 +161,8 @@ def lib():
         "anufft_debug_num_bins": (i32, [vp, P(i64)]),
         "anufft_debug_sort": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
         "anufft_debug_spread": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
+        "anufft_timing_enable": (i32, [vp, i32]),
+        "anufft_timing_collect": (i32, [vp, P(ctypes.c_double), P(i64)]),
         "anufft_last_error": (ctypes.c_char_p, []),
         "anufft_version": (ctypes.c_char_p, []),
     }

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -156,6 +157,28 @@ std::vector<int> factorize(int L) {
 
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Per-stage HIP-event timing (diagnostic; enabled per plan).
+struct StageTimer {
+  std::mutex mu;
+  bool on = false;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> pool;
+  std::vector<Rec> recs;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
          sub_ofs = 0, subs = 0, grid = 0, fftbuf = 0, total = 0;
@@ -187,6 +210,7 @@ struct anufft_plan {
   int max_sub = 1024;
   int lds_max = 163840; // gfx950 LDS per workgroup
   hipStream_t host_stream = nullptr;
+  mutable StageTimer timer;
 
   int64_t modes() const { return n[0] * n[1] * n[2]; }
   int64_t cells() const { return m[0] * m[1] * m[2]; }
@@ -302,6 +326,11 @@ void destroy_plan(anufft_plan *p) {
       if (p->d_tw[a]) (void)hipFree(p->d_tw[a]);
     }
     if (p->host_stream) (void)hipStreamDestroy(p->host_stream);
+    for (auto &r : p->timer.recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (auto e : p->timer.pool) (void)hipEventDestroy(e);
   }
   delete p;
 }
@@ -342,18 +371,43 @@ void choose_bins(anufft_plan *p) {
     p->passes = 1;
     return;
   }
-  const int cand[] = {32, 24, 16, 12, 8, 6, 4, 2, 1};
-  int user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
-  for (int c : cand) {
+  // Search tile shapes: maximise interior / padded volume (flush cost per
+  // point) subject to the LDS of one workgroup, then prefer larger tiles.
+  const int cand[] = {32, 24, 16, 12, 10, 8, 6, 4, 3, 2, 1};
+  const bool user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
+  auto set = [&](const int b[3]) {
     for (int a = 0; a < 3; ++a) {
-      int b = user ? p->opts.bin_size[a] : c;
-      g.bin[a] = std::min(b, g.m[a]);
+      g.bin[a] = std::min(b[a], g.m[a]);
       g.nbin[a] = (g.m[a] + g.bin[a] - 1) / g.bin[a];
       g.pad[a] = g.bin[a] + 2 * p->w;
     }
-    size_t need = std::max(spread_lds_bytes(p->w, g),
-                           std::max(interp_lds_bytes(p->w, g, false), interp_lds_bytes(p->w, g, true)));
-    if (user || need <= (size_t)p->lds_max) break;
+  };
+  auto lds_need = [&]() {
+    return std::max(spread_lds_bytes(p->w, g),
+                    std::max(interp_lds_bytes(p->w, g, false), interp_lds_bytes(p->w, g, true)));
+  };
+  if (user) {
+    set(p->opts.bin_size);
+  } else {
+    double best = -1.0;
+    int bb[3] = {1, 1, 1};
+    for (int bx : cand)
+      for (int by : cand)
+        for (int bz : cand) {
+          const int b[3] = {bx, by, bz};
+          set(b);
+          if (lds_need() > (size_t)p->lds_max) continue;
+          double in = (double)g.bin[0] * g.bin[1] * g.bin[2];
+          double pad = (double)g.pad[0] * g.pad[1] * g.pad[2];
+          double score = in / pad + 1e-9 * in;
+          if (score > best) {
+            best = score;
+            bb[0] = bx;
+            bb[1] = by;
+            bb[2] = bz;
+          }
+        }
+    set(bb);
   }
   p->nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
   int bits = 1;
@@ -455,6 +509,19 @@ struct Exec {
   hipStream_t s;
   char *ws;
   Layout lay;
+  // Run f, bracketed by events on the execution stream when timing is on.
+  template <typename F>
+  hipError_t timed(int stage, F f) const {
+    StageTimer &t = p->timer;
+    if (!t.on) return f();
+    std::lock_guard<std::mutex> lk(t.mu);
+    hipEvent_t a = t.get(), b = t.get();
+    if (a) (void)hipEventRecord(a, s);
+    hipError_t e = f();
+    if (b) (void)hipEventRecord(b, s);
+    if (a && b) t.recs.push_back({stage, a, b});
+    return e;
+  }
   float2 *grid() const { return (float2 *)(ws + lay.grid); }
   float2 *fftbuf() const { return (float2 *)(ws + lay.fftbuf); }
   SortBuffers sort_buffers() const {
@@ -691,22 +758,26 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   if (st != ANUFFT_OK) return st;
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
-  hipError_t e = launch_fill(ex.grid(), p->cells(), ex.s);
+  hipError_t e = ex.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ex.grid(), p->cells(), ex.s); });
   if (e != hipSuccess) return hip_fail(e, "fill");
   if (M > 0) {
     if (p->dim == 3) {
       PointArgs pa = ex.point_args(M, coords, stride);
-      e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+      e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s); });
       if (e != hipSuccess) return hip_fail(e, "binning");
-      e = launch_spread3d(p->w, pa, (const float2 *)values, ex.grid(), p->geom, p->poly,
-                          ex.spread_blocks(M), ex.s);
+      e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
+        return launch_spread3d(p->w, pa, (const float2 *)values, ex.grid(), p->geom, p->poly,
+                               ex.spread_blocks(M), ex.s);
+      });
     } else {
-      e = launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(), p->geom,
-                          p->poly, ex.s);
+      e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
+        return launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(),
+                               p->geom, p->poly, ex.s);
+      });
     }
     if (e != hipSuccess) return hip_fail(e, "spread");
   }
-  e = ex.grid_to_modes((float2 *)out);
+  e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.grid_to_modes((float2 *)out); });
   if (e != hipSuccess) return hip_fail(e, "fft");
   return ANUFFT_OK;
 }
@@ -723,17 +794,23 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
   hipError_t e;
   if (p->dim == 3) {
     PointArgs pa = ex.point_args(M, coords, stride);
-    e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
-    e = ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
+    e = ex.timed(ANUFFT_STAGE_FFT, [&] {
+      return ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
+    });
     if (e != hipSuccess) return hip_fail(e, "fft");
-    e = launch_interp3d(p->w, pa, ex.grid(), (float2 *)out, p->geom, p->poly,
-                        ex.spread_blocks(M), ex.s);
+    e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
+      return launch_interp3d(p->w, pa, ex.grid(), (float2 *)out, p->geom, p->poly,
+                             ex.spread_blocks(M), ex.s);
+    });
   } else {
-    e = ex.modes_to_grid((const float2 *)modes, false);
+    e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
-    e = launch_interp1d(p->w, coords[0], stride, M, ex.grid(), (float2 *)out, p->geom, p->poly,
-                        ex.s);
+    e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
+      return launch_interp1d(p->w, coords[0], stride, M, ex.grid(), (float2 *)out, p->geom,
+                             p->poly, ex.s);
+    });
   }
   if (e != hipSuccess) return hip_fail(e, "interp");
   return ANUFFT_OK;
@@ -755,19 +832,60 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
   // the derivative always uses the unnormalized (direct-sum) scaling
   if (p->dim == 3) {
     PointArgs pa = ex.point_args(M, coords, stride);
-    e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
-    e = ex.modes_to_grid((const float2 *)modes_h, false);
+    e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
-    e = launch_grad3d(p->w, pa, ex.grid(), (const float2 *)u, grad[0], grad[1], grad[2], p->geom,
-                      p->poly, ex.spread_blocks(M), ex.s);
+    e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
+      return launch_grad3d(p->w, pa, ex.grid(), (const float2 *)u, grad[0], grad[1], grad[2],
+                           p->geom, p->poly, ex.spread_blocks(M), ex.s);
+    });
   } else {
-    e = ex.modes_to_grid((const float2 *)modes_h, false);
+    e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
-    e = launch_grad1d(p->w, coords[0], stride, M, ex.grid(), (const float2 *)u, grad[0], p->geom,
-                      p->poly, ex.s);
+    e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
+      return launch_grad1d(p->w, coords[0], stride, M, ex.grid(), (const float2 *)u, grad[0],
+                           p->geom, p->poly, ex.s);
+    });
   }
   if (e != hipSuccess) return hip_fail(e, "grad");
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_timing_enable(const anufft_plan *p, int32_t on) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  std::lock_guard<std::mutex> lk(p->timer.mu);
+  for (auto &r : p->timer.recs) {
+    p->timer.pool.push_back(r.a);
+    p->timer.pool.push_back(r.b);
+  }
+  p->timer.recs.clear();
+  p->timer.on = on != 0;
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_timing_collect(const anufft_plan *p, double *ms, int64_t *calls) {
+  if (!p || !ms || !calls) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->timer.mu);
+  for (int i = 0; i < ANUFFT_STAGE_COUNT; ++i) {
+    ms[i] = 0.0;
+    calls[i] = 0;
+  }
+  hipError_t err = hipSuccess;
+  for (auto &r : p->timer.recs) {
+    float t = 0.f;
+    hipError_t e = hipEventSynchronize(r.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+    if (e != hipSuccess) err = e;
+    else if (r.stage >= 0 && r.stage < ANUFFT_STAGE_COUNT) {
+      ms[r.stage] += (double)t;
+      calls[r.stage] += 1;
+    }
+    p->timer.pool.push_back(r.a);
+    p->timer.pool.push_back(r.b);
+  }
+  p->timer.recs.clear();
+  if (err != hipSuccess) return hip_fail(err, "timing_collect");
   return ANUFFT_OK;
 }
 

@@ -15,10 +15,13 @@
 //   2. spread: each wave takes 16 points at once and maps its lanes to
 //      (point, y-tap, z-tap) triples (16 (2w)^2 = 64 w^2 items, so every
 //      lane is busy for every even tap count), loops over the 2w x-taps
-//      and adds into the padded tile with LDS float atomics (split re/im
-//      planes, consecutive z-taps on consecutive banks);
-//   3. flush: the padded tile (bin + 2w per axis) is added to the HBM grid
-//      with global float atomics, skipping zero cells, periodic wrap.
+//      and adds into the padded tile with LDS FP64 atomics (split re/im
+//      planes, consecutive z-taps on consecutive banks).  On gfx950 the
+//      LDS FP64 add runs ~23x faster than the LDS FP32 add (measured
+//      7.8 vs 0.33 lane-ops/clk/CU), and the tile accumulates in FP64;
+//   3. flush: the padded tile (bin + 2w per axis) is rounded to FP32 and
+//      added to the HBM grid with global float atomics, skipping zero
+//      cells, periodic wrap.
 // Interpolation loads the padded tile from HBM once, each wave gathers one
 // point at a time (lanes over (y,z) taps, loop over x) and reduces across
 // the wave.
@@ -27,7 +30,8 @@
 namespace anufft {
 namespace {
 
-constexpr int kStage = 128;    // points staged per round
+constexpr int kStage = 128;       // points staged per round (interp, grad)
+constexpr int kSpreadStage = 64;  // points staged per round (spread)
 constexpr int kSpreadThreads = 512;
 constexpr int kInterpThreads = 256;
 
@@ -96,17 +100,17 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
   constexpr int T = 2 * W;
   constexpr int WS = Staging<W>::WS;
   constexpr int NT = kSpreadThreads;
-  extern __shared__ float sh[];
+  extern __shared__ double shd[];
   const int Px = g.pad[0], Py = g.pad[1], Pz = g.pad[2];
   const int P = Px * Py * Pz;
   const int PyPz = Py * Pz;
-  float *tre = sh;
-  float *tim = sh + P;
-  float *wts = sh + 2 * P;
-  float2 *cv = (float2 *)(wts + ((kStage * WS + 1) & ~1));
-  int *loc = (int *)(cv + kStage);
+  double *tre = shd;
+  double *tim = shd + P;
+  float *wts = (float *)(shd + 2 * P);
+  float2 *cv = (float2 *)(wts + ((kSpreadStage * WS + 1) & ~1));
+  int *loc = (int *)(cv + kSpreadStage);
 
-  for (int i = threadIdx.x; i < 2 * P; i += NT) sh[i] = 0.f;
+  for (int i = threadIdx.x; i < 2 * P; i += NT) shd[i] = 0.0;
   __syncthreads();
 
   const uint32_t nsub = *pa.nsub;
@@ -120,8 +124,8 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
     const int bx = bin / (g.nbin[2] * g.nbin[1]);
     const int ox = bx * g.bin[0] - W, oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
 
-    for (int c0 = 0; c0 < sub.z; c0 += kStage) {
-      const int npts = min(kStage, sub.z - c0);
+    for (int c0 = 0; c0 < sub.z; c0 += kSpreadStage) {
+      const int npts = min(kSpreadStage, sub.z - c0);
       stage_points<W, NT, false>(pa, g, poly, values, sub.y + c0, npts, ox, oy, oz, wts,
                                  nullptr, cv, loc, nullptr);
       __syncthreads();
@@ -141,8 +145,8 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
 #pragma unroll
             for (int tx = 0; tx < T; ++tx) {
               const float wx = wp[tx];
-              atomicAdd(&tre[base + tx * PyPz], cr * wx);
-              atomicAdd(&tim[base + tx * PyPz], ci * wx);
+              atomicAdd(&tre[base + tx * PyPz], (double)(cr * wx));
+              atomicAdd(&tim[base + tx * PyPz], (double)(ci * wx));
             }
           }
         }
@@ -152,10 +156,10 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
 
     // flush the padded tile into the grid and clear it
     for (int i = threadIdx.x; i < P; i += NT) {
-      const float re = tre[i], im = tim[i];
-      if (re != 0.f || im != 0.f) {
-        tre[i] = 0.f;
-        tim[i] = 0.f;
+      const double re = tre[i], im = tim[i];
+      if (re != 0.0 || im != 0.0) {
+        tre[i] = 0.0;
+        tim[i] = 0.0;
         const int pz = i % Pz;
         const int t2 = i / Pz;
         const int py = t2 % Py;
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
         const int gy = wrap(oy + py, g.m[1]);
         const int gz = wrap(oz + pz, g.m[2]);
         float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
-        unsafeAtomicAdd(&dst->x, re);
-        unsafeAtomicAdd(&dst->y, im);
+        unsafeAtomicAdd(&dst->x, (float)re);
+        unsafeAtomicAdd(&dst->y, (float)im);
       }
     }
     __syncthreads();
@@ -420,8 +424,9 @@ __global__ void grad1d_kernel(const float *x, int64_t stride, int64_t n, const f
 template <int W>
 size_t spread_lds(const Geom &g) {
   const size_t P = (size_t)g.pad[0] * g.pad[1] * g.pad[2];
-  const size_t ws = ((size_t)kStage * Staging<W>::WS + 1) & ~(size_t)1;
-  return (2 * P + ws) * sizeof(float) + kStage * sizeof(float2) + 3 * kStage * sizeof(int);
+  const size_t ws = ((size_t)kSpreadStage * Staging<W>::WS + 1) & ~(size_t)1;
+  return 2 * P * sizeof(double) + ws * sizeof(float) + kSpreadStage * sizeof(float2) +
+         3 * kSpreadStage * sizeof(int);
 }
 
 template <int W>

@@ -93,6 +93,17 @@ class _PlanBase:
             self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=device)
         return self._ws
 
+    def timing(self, on=True):
+        """Enable/disable per-stage HIP-event timing (clears records)."""
+        N.check(N.lib().anufft_timing_enable(self._h, int(bool(on))), "timing_enable")
+
+    def timing_collect(self):
+        """{stage: (total_ms, calls)} of the recorded executions; clears them."""
+        ms = (ctypes.c_double * len(N.STAGES))()
+        calls = (ctypes.c_int64 * len(N.STAGES))()
+        N.check(N.lib().anufft_timing_collect(self._h, ms, calls), "timing_collect")
+        return {name: (ms[i], calls[i]) for i, name in enumerate(N.STAGES)}
+
     def close(self):
         if self._h:
             N.lib().anufft_plan_destroy(self._h)

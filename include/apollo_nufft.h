@@ -201,6 +201,24 @@ anufft_status anufft_execute_type2_host(const anufft_plan *plan, int32_t storage
                                         const double *const coords[3], int64_t coord_stride,
                                         const void *modes, void *out);
 
+/* Per-stage timing with HIP events recorded on the execution stream
+ * (no reference counterpart; the reference records no timings, every WGPU
+ * pass sets timestamp_writes: None, kernel.rs:876).  While enabled, each
+ * execute call records an event pair around each stage; collect waits for
+ * them, returns the summed milliseconds and call counts per stage and clears
+ * the records.  Not for concurrent use of one plan from several threads. */
+typedef enum anufft_stage {
+  ANUFFT_STAGE_FILL = 0,   /* zeroing of the oversampled grid */
+  ANUFFT_STAGE_BIN = 1,    /* point binning (counting sort) and subproblems */
+  ANUFFT_STAGE_SPREAD = 2, /* Type-1 spreading */
+  ANUFFT_STAGE_FFT = 3,    /* all FFT passes with pad/crop/deconvolution */
+  ANUFFT_STAGE_INTERP = 4, /* Type-2 interpolation */
+  ANUFFT_STAGE_GRAD = 5,   /* coordinate-gradient interpolation */
+  ANUFFT_STAGE_COUNT = 6
+} anufft_stage;
+anufft_status anufft_timing_enable(const anufft_plan *plan, int32_t on);
+anufft_status anufft_timing_collect(const anufft_plan *plan, double *ms, int64_t *calls);
+
 /* Diagnostics for stage-level parity tests (no reference counterpart).
  * anufft_debug_sort: the point binning of a 3D plan -- perm[i] is the input
  * index of the i-th point in tile order (stable: ties keep input order),

@@ -1,0 +1,150 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol the
+header declares, plan metadata matches the oracle's restatement of
+NufftPlan{1D,3D}::new, validation errors map to the reference's error kinds,
+and host-side argument checks fire before any device work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import apollofft_amd as A
+import apollofft_amd._native as N
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    with open(os.path.join(ROOT, "include", "apollo_nufft.h")) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(anufft_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_symbol():
+    lib = N.lib()
+    syms = _header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(N.EXPORTS)
+    assert lib.anufft_version().decode().startswith("apollo-nufft-hip")
+
+
+def _d(n):
+    return 2 * np.pi / n
+
+
+@pytest.mark.parametrize("n,w,policy", [((64, 64, 64), 7, "reference"), ((100, 100, 100), 8, "exact"),
+                                        ((100, 100, 100), 4, "reference"), ((127, 96, 48), 6, "exact"),
+                                        ((5, 3, 2), 2, "reference")])
+def test_plan3d_metadata_matches_oracle(n, w, policy):
+    p = A.NufftPlan3D(*n, _d(n[0]), _d(n[1]), _d(n[2]), 2, w, grid_policy=policy)
+    m = p.meta
+    pol = O.POLICY_EXACT if policy == "exact" else O.POLICY_POW2
+    assert m["m"] == tuple(O.oversampled_len(k, 2, w, pol) for k in n)
+    assert m["beta"] == O.beta(2, w)
+    assert m["i0_beta"] == O.i0(O.beta(2, w))
+    assert m["kernel_width"] == w and m["sigma"] == 2 and m["dim"] == 3
+    for a in range(3):
+        ref = O.axis_deconv(n[a], m["m"][a], w, m["beta"], m["i0_beta"])
+        assert np.allclose(p.deconv(a), ref, rtol=1e-15, atol=0)
+    assert m["poly_max_err"] < 5e-8
+    # tiles fit the gfx950 workgroup LDS and cover the grid
+    for a in range(3):
+        assert 1 <= m["bin_size"][a] <= m["m"][a]
+
+
+def test_plan1d_metadata_matches_oracle():
+    p = A.NufftPlan1D(256, _d(256), 2, 6)
+    m = p.meta
+    assert m["m"][0] == O.oversampled_len(256, 2, 6, O.POLICY_1D) == 512
+    assert np.allclose(p.deconv(0), O.axis_deconv(256, 512, 6, m["beta"], m["i0_beta"]), rtol=1e-15)
+    # sigma n without the 2w+1 floor (dimension_1d.rs:131)
+    assert A.NufftPlan1D(4, 0.25, 2, 6).meta["m"][0] == 8
+
+
+@pytest.mark.parametrize("w", range(2, 9))
+def test_kernel_polynomial_fit_accuracy(w):
+    p = A.NufftPlan3D(16, 16, 16, 0.1, 0.1, 0.1, 2, w)
+    assert p.meta["poly_max_err"] < 5e-8
+    assert p.meta["poly_degree"] == 10
+
+
+def test_validation_errors():
+    with pytest.raises(A.InvalidArgument, match="nx"):
+        A.NufftPlan3D(0, 4, 4, 0.1, 0.1, 0.1)
+    with pytest.raises(A.InvalidArgument, match="dy"):
+        A.NufftPlan3D(4, 4, 4, 0.1, -0.1, 0.1)
+    with pytest.raises(A.InvalidArgument, match="dz"):
+        A.NufftPlan3D(4, 4, 4, 0.1, 0.1, float("nan"))
+    with pytest.raises(A.InvalidArgument, match="sigma"):
+        A.NufftPlan3D(4, 4, 4, 0.1, 0.1, 0.1, 1, 6)
+    with pytest.raises(A.InvalidArgument, match="kernel_width"):
+        A.NufftPlan1D(8, 0.1, 2, 1)
+    with pytest.raises(A.InvalidArgument, match="field n"):
+        A.NufftPlan1D(0, 0.1)
+    with pytest.raises(A.InvalidArgument, match="dx"):
+        A.NufftPlan1D(8, float("inf"))
+    with pytest.raises(A.Unsupported):
+        A.NufftPlan3D(4, 4, 4, 0.1, 0.1, 0.1, 2, 9)
+    with pytest.raises(A.Unsupported):
+        A.NufftPlan1D(4096, 0.1, 2, 6)  # m = 8192 exceeds the in-LDS FFT line
+    with pytest.raises(A.InvalidArgument):
+        A.NufftPlan3D(64, 64, 64, 0.1, 0.1, 0.1, 2, 7, bin_size=32)  # LDS overflow
+
+
+def test_workspace_grows_with_points():
+    p = A.NufftPlan3D(32, 32, 32, _d(32), _d(32), _d(32), 2, 6)
+    sizes = [p.workspace_size(M) for M in (0, 1, 1000, 10 ** 6)]
+    assert sizes == sorted(sizes) and sizes[0] > 0
+    # the fine grid alone is 64^3 complex64
+    assert sizes[0] >= 64 ** 3 * 8
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except ImportError:
+        return False
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the device-less error path")
+def test_execute_without_device_reports_unavailable():
+    p = A.NufftPlan1D(8, 0.125)
+    with pytest.raises(A.DeviceUnavailable):
+        p.type1_host(np.zeros(3), np.zeros(3, complex))
+    lib = N.lib()
+    st = lib.anufft_execute_type1(p.handle, 0, N.ptr_array([None] * 3), 1, None, None, None, 0, None)
+    assert st in (N.INVALID_ARG, N.DEVICE_UNAVAILABLE)
+
+
+def test_c_abi_null_and_capacity_errors():
+    lib = N.lib()
+    assert lib.anufft_plan_get_meta(None, None) == N.INVALID_ARG
+    assert "NULL" in lib.anufft_last_error().decode()
+    h = ctypes.c_void_p()
+    o = N.default_opts()
+    assert lib.anufft_plan3d_create(4, 4, 4, 0.1, 0.1, 0.1, 2, 6, ctypes.byref(o), None) == N.INVALID_ARG
+    assert lib.anufft_plan3d_create(4, 4, 4, 0.1, 0.1, 0.1, 2, 6, ctypes.byref(o), ctypes.byref(h)) == N.OK
+    m = N.Meta()
+    assert lib.anufft_plan_get_meta(h, ctypes.byref(m)) == N.OK
+    out = np.empty(4)
+    assert lib.anufft_plan_get_deconv(h, 3, out.ctypes.data) == N.INVALID_ARG
+    b = ctypes.c_size_t()
+    assert lib.anufft_workspace_size(h, -1, ctypes.byref(b)) == N.INVALID_ARG
+    assert lib.anufft_plan_destroy(h) == N.OK
+
+
+def test_python_api_argument_checks():
+    pos = np.zeros((10, 3))
+    with pytest.raises(ValueError, match="C-contiguous"):
+        A.nufft_type1_3d_fast(pos[::2], np.zeros(5, complex), 4, 4, 4, 0.1, 0.1, 0.1)
+    with pytest.raises(ValueError, match="shape"):
+        A.nufft_type1_3d_fast(np.zeros((10, 2)), np.zeros(10, complex), 4, 4, 4, 0.1, 0.1, 0.1)
+    with pytest.raises(ValueError, match="mismatch"):
+        A.nufft_type1_3d_fast(pos, np.zeros(9, complex), 4, 4, 4, 0.1, 0.1, 0.1)
+    with pytest.raises(ValueError, match="mismatch"):
+        A.nufft_type1_1d_fast(np.zeros(4), np.zeros(3, complex), 0.1)
