@@ -118,6 +118,8 @@ class Meta(ctypes.Structure):
         ("compat_reference_type2_3d_scaling", ctypes.c_int32),
         ("bin_size", ctypes.c_int32 * 3),
         ("max_subproblem", ctypes.c_int32),
+        ("interp_bin_size", ctypes.c_int32 * 3),
+        ("interp_max_subproblem", ctypes.c_int32),
         ("poly_degree", ctypes.c_int32),
         ("poly_max_err", ctypes.c_double),
     ]

@@ -5,8 +5,8 @@
 //   modes       float2[nx][ny][nz]   (FFT order, k <= n/2 positive)
 //   points      float coords[a][j*stride], float2 values[j]
 // The kernel phi is evaluated from per-tap polynomials: for a point at grid
-// coordinate t, the first tap is i0 = ceil(t - w), x1 = i0 - t in [-w,-w+1),
-// z = 2(x1 + w) - 1 in [-1,1), and tap i has phi(x1 + i) = P_i(z).
+// coordinate t in cell c, the first tap is i0 = c - w + 1, x1 = i0 - t in
+// [-w,-w+1], z = 2(x1 + w) - 1 in [-1,1], and tap i has phi(x1 + i) = P_i(z).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +25,8 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;       // points per thread per chunk
 constexpr int kSortChunk = kSortThreads * kSortItems;
+constexpr int kSpreadBinX = 4;      // spreading tile x-extent (a lane holds bin_x + 2w - 1 cells)
+constexpr int kSpreadCols = 32;     // spreading tile (y,z) columns per axis, bin + 2w <= 32
 
 // Per-tap kernel polynomials, passed by value (kernarg segment -> scalar regs).
 // c[d][i] multiplies z^d for tap i; dc[d][i] is the same for d phi/dx.
@@ -136,12 +138,17 @@ __device__ __forceinline__ double fold_coord(float xf, double scale, int m, int 
   return t;
 }
 
-// Evaluate the 2w non-zero tap weights of one axis for grid coordinate t.
-// Returns the first tap index i0 = ceil(t - w).
+// Evaluate the 2w tap weights of one axis for grid coordinate t in cell c
+// (t in [c, c+1]).  The taps are i0 + i, i0 = c - w + 1, i in [0, 2w), at
+// distances x1 + i with x1 = i0 - t in [-w, -w+1]; with z = 2(x1 + w) - 1 =
+// 1 - 2(t - c) the weight of tap i is P_i(z).  These are exactly the
+// non-zero taps of the reference loop p in [-w, w] around round(t)
+// (dimension_3d.rs:294-305): a tap at distance +-w has phi = 0
+// (kaiser_bessel.rs:111-112), which the two endpoint cases enforce.  All
+// points of one cell share i0.  Returns i0.
 template <int W>
-__device__ __forceinline__ int kb_taps(double t, const KbPoly &P, float *wt) {
-  double i0d = ceil(t - (double)W);
-  float z = (float)(2.0 * (i0d - t + (double)W) - 1.0);
+__device__ __forceinline__ int kb_taps(double t, int cell, const KbPoly &P, float *wt) {
+  const float z = (float)(1.0 - 2.0 * (t - (double)cell));
 #pragma unroll
   for (int i = 0; i < 2 * W; ++i) {
     float v = P.c[kPolyCoefs - 1][i];
@@ -149,15 +156,16 @@ __device__ __forceinline__ int kb_taps(double t, const KbPoly &P, float *wt) {
     for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, P.c[d][i]);
     wt[i] = v;
   }
-  if (z <= -1.0f) wt[0] = 0.0f; // x1 == -w: phi(-w) = 0 (kaiser_bessel.rs:111-112)
-  return (int)i0d;
+  if (z <= -1.0f) wt[0] = 0.0f;          // t = c + 1: first tap at distance -w
+  if (z >= 1.0f) wt[2 * W - 1] = 0.0f;   // t = c: last tap at distance +w
+  return cell - W + 1;
 }
 
 // Value and derivative (d phi / d x, per grid unit) taps.
 template <int W>
-__device__ __forceinline__ int kb_taps_deriv(double t, const KbPoly &P, float *wt, float *dwt) {
-  double i0d = ceil(t - (double)W);
-  float z = (float)(2.0 * (i0d - t + (double)W) - 1.0);
+__device__ __forceinline__ int kb_taps_deriv(double t, int cell, const KbPoly &P, float *wt,
+                                             float *dwt) {
+  const float z = (float)(1.0 - 2.0 * (t - (double)cell));
 #pragma unroll
   for (int i = 0; i < 2 * W; ++i) {
     float v = P.c[kPolyCoefs - 1][i];
@@ -174,7 +182,11 @@ __device__ __forceinline__ int kb_taps_deriv(double t, const KbPoly &P, float *w
     wt[0] = 0.0f;
     dwt[0] = 0.0f;
   }
-  return (int)i0d;
+  if (z >= 1.0f) {
+    wt[2 * W - 1] = 0.0f;
+    dwt[2 * W - 1] = 0.0f;
+  }
+  return cell - W + 1;
 }
 #endif
 

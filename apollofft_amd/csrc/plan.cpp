@@ -204,10 +204,14 @@ struct anufft_plan {
   std::vector<int> fac[3];
   KbPoly poly{};
   double poly_err = 0;
-  Geom geom{};
-  int nbins = 1;
-  int passes = 1;
-  int max_sub = 1024;
+  // Binning geometries: spreading (register columns) and interpolation /
+  // gradient (LDS tile).  Each execute sorts with the geometry it uses.
+  struct Binning {
+    Geom g{};
+    int nbins = 1;
+    int passes = 1;
+    int max_sub = 1024;
+  } bs, bi;
   int lds_max = 163840; // gfx950 LDS per workgroup
   hipStream_t host_stream = nullptr;
   mutable StageTimer timer;
@@ -247,11 +251,13 @@ Layout layout(const anufft_plan *p, int64_t M) {
     l.idx1 = take(4 * Mp);
     int64_t cnt_n = (int64_t)kRadix * nch;
     l.cnt = take(4 * (size_t)cnt_n);
-    int64_t scan_n = std::max<int64_t>(cnt_n, p->nbins + 1);
+    int64_t scan_n = std::max<int64_t>(cnt_n, std::max(p->bs.nbins, p->bi.nbins) + 1);
     l.scan = take(4 * (size_t)((scan_n + 1023) / 1024 + 1));
-    l.bin_start = take(4 * (size_t)(p->nbins + 1));
-    l.sub_ofs = take(4 * (size_t)(p->nbins + 1));
-    int64_t nsub = p->nbins + (M + p->max_sub - 1) / p->max_sub + 1;
+    const int nb = std::max(p->bs.nbins, p->bi.nbins);
+    l.bin_start = take(4 * (size_t)(nb + 1));
+    l.sub_ofs = take(4 * (size_t)(nb + 1));
+    int64_t nsub = std::max<int64_t>(p->bs.nbins + (M + p->bs.max_sub - 1) / p->bs.max_sub,
+                                     p->bi.nbins + (M + p->bi.max_sub - 1) / p->bi.max_sub) + 1;
     l.subs = take(16 * (size_t)nsub);
     l.grid = take(8 * (size_t)p->cells());
     l.fftbuf = take(8 * (size_t)(p->m[0] * p->m[1] * p->n[2]));
@@ -353,52 +359,56 @@ void init_device(anufft_plan *p) {
     p->lds_max = v;
 }
 
-// Tile sizes: largest cube edge whose spread, interp and gradient kernels fit
-// the LDS of one workgroup, clipped to the grid.
-void choose_bins(anufft_plan *p) {
-  Geom &g = p->geom;
+// Binning geometries.  Spreading: tiles of kSpreadBinX x (32-2w) x (32-2w)
+// cells, so that the padded (y,z) extent matches the 32 x 32 lane columns of
+// the spreading workgroup.  Interpolation: the tile shape whose padded LDS
+// copy (and the gradient kernel's) fits one workgroup's LDS and maximises
+// interior / padded volume.
+void set_geom(anufft_plan::Binning &b, const anufft_plan *p, const int bin[3]) {
+  Geom &g = b.g;
   for (int a = 0; a < 3; ++a) {
     g.m[a] = (int)p->m[a];
     g.scale[a] = (double)p->m[a] / p->L[a];
+    g.bin[a] = std::max(1, std::min(bin[a], g.m[a]));
+    g.nbin[a] = (g.m[a] + g.bin[a] - 1) / g.bin[a];
+    g.pad[a] = g.bin[a] + 2 * p->w;
   }
+  b.nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
+  int bits = 1;
+  while ((1ll << bits) < (long long)b.nbins) ++bits;
+  b.passes = (bits + kRadixBits - 1) / kRadixBits;
+}
+
+void choose_bins(anufft_plan *p) {
   if (p->dim == 1) {
-    for (int a = 0; a < 3; ++a) {
-      g.bin[a] = g.m[a];
-      g.nbin[a] = 1;
-      g.pad[a] = g.m[a];
-    }
-    p->nbins = 1;
-    p->passes = 1;
+    const int b[3] = {(int)p->m[0], 1, 1};
+    set_geom(p->bs, p, b);
+    set_geom(p->bi, p, b);
+    p->bs.nbins = p->bi.nbins = 1;
     return;
   }
-  // Search tile shapes: maximise interior / padded volume (flush cost per
-  // point) subject to the LDS of one workgroup, then prefer larger tiles.
-  const int cand[] = {32, 24, 16, 12, 10, 8, 6, 4, 3, 2, 1};
   const bool user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
-  auto set = [&](const int b[3]) {
-    for (int a = 0; a < 3; ++a) {
-      g.bin[a] = std::min(b[a], g.m[a]);
-      g.nbin[a] = (g.m[a] + g.bin[a] - 1) / g.bin[a];
-      g.pad[a] = g.bin[a] + 2 * p->w;
-    }
-  };
-  auto lds_need = [&]() {
-    return std::max(spread_lds_bytes(p->w, g),
-                    std::max(interp_lds_bytes(p->w, g, false), interp_lds_bytes(p->w, g, true)));
-  };
+  const int cyz = kSpreadCols - 2 * p->w;
+  const int bsp[3] = {kSpreadBinX, cyz, cyz};
+  set_geom(p->bs, p, bsp);
+  p->bs.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 4096;
+
+  const int cand[] = {32, 24, 16, 12, 10, 8, 6, 4, 3, 2, 1};
   if (user) {
-    set(p->opts.bin_size);
+    set_geom(p->bi, p, p->opts.bin_size);
   } else {
     double best = -1.0;
     int bb[3] = {1, 1, 1};
+    anufft_plan::Binning t;
     for (int bx : cand)
       for (int by : cand)
         for (int bz : cand) {
           const int b[3] = {bx, by, bz};
-          set(b);
-          if (lds_need() > (size_t)p->lds_max) continue;
-          double in = (double)g.bin[0] * g.bin[1] * g.bin[2];
-          double pad = (double)g.pad[0] * g.pad[1] * g.pad[2];
+          set_geom(t, p, b);
+          size_t need = std::max(interp_lds_bytes(p->w, t.g, false), interp_lds_bytes(p->w, t.g, true));
+          if (need > (size_t)p->lds_max) continue;
+          double in = (double)t.g.bin[0] * t.g.bin[1] * t.g.bin[2];
+          double pad = (double)t.g.pad[0] * t.g.pad[1] * t.g.pad[2];
           double score = in / pad + 1e-9 * in;
           if (score > best) {
             best = score;
@@ -407,13 +417,9 @@ void choose_bins(anufft_plan *p) {
             bb[2] = bz;
           }
         }
-    set(bb);
+    set_geom(p->bi, p, bb);
   }
-  p->nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
-  int bits = 1;
-  while ((1ll << bits) < (long long)p->nbins) ++bits;
-  p->passes = (bits + kRadixBits - 1) / kRadixBits;
-  p->max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 1024;
+  p->bi.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 1024;
 }
 
 anufft_status finish_create(anufft_plan *p, anufft_plan **out) {
@@ -427,9 +433,9 @@ anufft_status finish_create(anufft_plan *p, anufft_plan **out) {
   p->i0b = kb_i0(p->beta);
   choose_bins(p);
   if (p->dim == 3) {
-    size_t need = std::max(spread_lds_bytes(p->w, p->geom),
-                           std::max(interp_lds_bytes(p->w, p->geom, false),
-                                    interp_lds_bytes(p->w, p->geom, true)));
+    size_t need = std::max(spread_lds_bytes(p->w, p->bs.g),
+                           std::max(interp_lds_bytes(p->w, p->bi.g, false),
+                                    interp_lds_bytes(p->w, p->bi.g, true)));
     if (need > (size_t)p->lds_max) {
       destroy_plan(p);
       return fail(ANUFFT_INVALID_ARG, "bin_size needs %zu bytes of LDS, device has %d", need,
@@ -524,7 +530,7 @@ struct Exec {
   }
   float2 *grid() const { return (float2 *)(ws + lay.grid); }
   float2 *fftbuf() const { return (float2 *)(ws + lay.fftbuf); }
-  SortBuffers sort_buffers() const {
+  SortBuffers sort_buffers(const anufft_plan::Binning &b) const {
     SortBuffers sb{};
     sb.keys[0] = (uint32_t *)(ws + lay.keys0);
     sb.keys[1] = (uint32_t *)(ws + lay.keys1);
@@ -535,29 +541,30 @@ struct Exec {
     sb.bin_start = (uint32_t *)(ws + lay.bin_start);
     sb.sub_ofs = (uint32_t *)(ws + lay.sub_ofs);
     sb.subs = (int4 *)(ws + lay.subs);
-    sb.nbins = p->nbins;
+    sb.nbins = b.nbins;
     sb.nchunks = lay.nchunks;
-    sb.passes = p->passes;
-    sb.max_sub = p->max_sub;
+    sb.passes = b.passes;
+    sb.max_sub = b.max_sub;
     return sb;
   }
-  PointArgs point_args(int64_t M, const float *const c[3], int64_t stride) const {
+  PointArgs point_args(const anufft_plan::Binning &b, int64_t M, const float *const c[3],
+                       int64_t stride) const {
     PointArgs pa{};
     pa.x = c[0];
     pa.y = p->dim == 3 ? c[1] : c[0];
     pa.z = p->dim == 3 ? c[2] : c[0];
     pa.stride = stride;
     pa.num_points = M;
-    SortBuffers sb = sort_buffers();
-    pa.perm = sb.idx[p->passes & 1];
+    SortBuffers sb = sort_buffers(b);
+    pa.perm = sb.idx[b.passes & 1];
     pa.subs = sb.subs;
-    pa.nsub = sb.sub_ofs + p->nbins;
+    pa.nsub = sb.sub_ofs + b.nbins;
     return pa;
   }
-  int spread_blocks(int64_t M) const {
-    int64_t up = (int64_t)p->nbins + (M + p->max_sub - 1) / p->max_sub;
-    int64_t b = std::min<int64_t>(up, 4 * (int64_t)p->cus);
-    return (int)std::max<int64_t>(b, 1);
+  int blocks(const anufft_plan::Binning &b, int64_t M, int per_cu) const {
+    int64_t up = (int64_t)b.nbins + (M + b.max_sub - 1) / b.max_sub;
+    int64_t n = std::min<int64_t>(up, (int64_t)per_cu * p->cus);
+    return (int)std::max<int64_t>(n, 1);
   }
 
   // T1 grid -> modes: forward passes z, y, x with crop + deconvolution.
@@ -722,7 +729,8 @@ anufft_status anufft_plan_get_meta(const anufft_plan *p, anufft_meta *meta) {
     meta->n[a] = p->n[a];
     meta->m[a] = p->m[a];
     meta->length[a] = a < p->dim ? p->L[a] : 0.0;
-    meta->bin_size[a] = p->geom.bin[a];
+    meta->bin_size[a] = p->bs.g.bin[a];
+    meta->interp_bin_size[a] = p->bi.g.bin[a];
   }
   meta->kernel_width = p->w;
   meta->sigma = p->sigma;
@@ -730,7 +738,8 @@ anufft_status anufft_plan_get_meta(const anufft_plan *p, anufft_meta *meta) {
   meta->i0_beta = p->i0b;
   meta->grid_policy = p->opts.grid_policy;
   meta->compat_reference_type2_3d_scaling = p->opts.compat_reference_type2_3d_scaling;
-  meta->max_subproblem = p->max_sub;
+  meta->max_subproblem = p->bs.max_sub;
+  meta->interp_max_subproblem = p->bi.max_sub;
   meta->poly_degree = kPolyCoefs - 1;
   meta->poly_max_err = p->poly_err;
   return ANUFFT_OK;
@@ -762,17 +771,17 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   if (e != hipSuccess) return hip_fail(e, "fill");
   if (M > 0) {
     if (p->dim == 3) {
-      PointArgs pa = ex.point_args(M, coords, stride);
-      e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s); });
+      PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+      e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bs.g, 3, ex.sort_buffers(p->bs), ex.s); });
       if (e != hipSuccess) return hip_fail(e, "binning");
       e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
-        return launch_spread3d(p->w, pa, (const float2 *)values, ex.grid(), p->geom, p->poly,
-                               ex.spread_blocks(M), ex.s);
+        return launch_spread3d(p->w, pa, (const float2 *)values, ex.grid(), p->bs.g, p->poly,
+                               ex.blocks(p->bs, M, 2), ex.s);
       });
     } else {
       e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
         return launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(),
-                               p->geom, p->poly, ex.s);
+                               p->bs.g, p->poly, ex.s);
       });
     }
     if (e != hipSuccess) return hip_fail(e, "spread");
@@ -793,22 +802,22 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
   hipError_t e;
   if (p->dim == 3) {
-    PointArgs pa = ex.point_args(M, coords, stride);
-    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s); });
+    PointArgs pa = ex.point_args(p->bi, M, coords, stride);
+    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bi.g, 3, ex.sort_buffers(p->bi), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
     e = ex.timed(ANUFFT_STAGE_FFT, [&] {
       return ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
     });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp3d(p->w, pa, ex.grid(), (float2 *)out, p->geom, p->poly,
-                             ex.spread_blocks(M), ex.s);
+      return launch_interp3d(p->w, pa, ex.grid(), (float2 *)out, p->bi.g, p->poly,
+                             ex.blocks(p->bi, M, 4), ex.s);
     });
   } else {
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp1d(p->w, coords[0], stride, M, ex.grid(), (float2 *)out, p->geom,
+      return launch_interp1d(p->w, coords[0], stride, M, ex.grid(), (float2 *)out, p->bi.g,
                              p->poly, ex.s);
     });
   }
@@ -831,21 +840,21 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
   hipError_t e;
   // the derivative always uses the unnormalized (direct-sum) scaling
   if (p->dim == 3) {
-    PointArgs pa = ex.point_args(M, coords, stride);
-    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s); });
+    PointArgs pa = ex.point_args(p->bi, M, coords, stride);
+    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bi.g, 3, ex.sort_buffers(p->bi), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
       return launch_grad3d(p->w, pa, ex.grid(), (const float2 *)u, grad[0], grad[1], grad[2],
-                           p->geom, p->poly, ex.spread_blocks(M), ex.s);
+                           p->bi.g, p->poly, ex.blocks(p->bi, M, 4), ex.s);
     });
   } else {
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
       return launch_grad1d(p->w, coords[0], stride, M, ex.grid(), (const float2 *)u, grad[0],
-                           p->geom, p->poly, ex.s);
+                           p->bi.g, p->poly, ex.s);
     });
   }
   if (e != hipSuccess) return hip_fail(e, "grad");
@@ -891,7 +900,7 @@ anufft_status anufft_timing_collect(const anufft_plan *p, double *ms, int64_t *c
 
 anufft_status anufft_debug_num_bins(const anufft_plan *p, int64_t *num_bins) {
   if (!p || !num_bins) return fail(ANUFFT_INVALID_ARG, "NULL argument");
-  *num_bins = p->nbins;
+  *num_bins = p->bs.nbins;
   return ANUFFT_OK;
 }
 
@@ -904,13 +913,13 @@ anufft_status anufft_debug_sort(const anufft_plan *p, int64_t M, const float *co
   if (p->dim != 3) return fail(ANUFFT_UNSUPPORTED, "1D plans do not bin points");
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
-  PointArgs pa = ex.point_args(M, coords, stride);
-  SortBuffers sb = ex.sort_buffers();
-  hipError_t e = sort_points(pa, p->geom, 3, sb, ex.s);
+  PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+  SortBuffers sb = ex.sort_buffers(p->bs);
+  hipError_t e = sort_points(pa, p->bs.g, 3, sb, ex.s);
   if (e != hipSuccess) return hip_fail(e, "binning");
   if (M > 0 && (e = hipMemcpyAsync(perm, pa.perm, 4 * (size_t)M, hipMemcpyDeviceToDevice, ex.s)) != hipSuccess)
     return hip_fail(e, "copy perm");
-  if ((e = hipMemcpyAsync(bin_start, sb.bin_start, 4 * (size_t)(p->nbins + 1), hipMemcpyDeviceToDevice, ex.s)) != hipSuccess)
+  if ((e = hipMemcpyAsync(bin_start, sb.bin_start, 4 * (size_t)(p->bs.nbins + 1), hipMemcpyDeviceToDevice, ex.s)) != hipSuccess)
     return hip_fail(e, "copy bin_start");
   return ANUFFT_OK;
 }
@@ -927,14 +936,14 @@ anufft_status anufft_debug_spread(const anufft_plan *p, int64_t M, const float *
   if (e != hipSuccess) return hip_fail(e, "fill");
   if (M == 0) return ANUFFT_OK;
   if (p->dim == 3) {
-    PointArgs pa = ex.point_args(M, coords, stride);
-    e = sort_points(pa, p->geom, 3, ex.sort_buffers(), ex.s);
+    PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+    e = sort_points(pa, p->bs.g, 3, ex.sort_buffers(p->bs), ex.s);
     if (e != hipSuccess) return hip_fail(e, "binning");
-    e = launch_spread3d(p->w, pa, (const float2 *)values, (float2 *)grid, p->geom, p->poly,
-                        ex.spread_blocks(M), ex.s);
+    e = launch_spread3d(p->w, pa, (const float2 *)values, (float2 *)grid, p->bs.g, p->poly,
+                        ex.blocks(p->bs, M, 2), ex.s);
   } else {
     e = launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, (float2 *)grid,
-                        p->geom, p->poly, ex.s);
+                        p->bs.g, p->poly, ex.s);
   }
   if (e != hipSuccess) return hip_fail(e, "spread");
   return ANUFFT_OK;

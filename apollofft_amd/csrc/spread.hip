@@ -7,33 +7,35 @@
 // fast_type1_spread_3d / fast_type2_interpolate_3d (apollo-nufft-wgpu
 // .../shaders/nufft_fast_3d.wgsl:130-161, :268-299).
 //
-// 3D work unit = subproblem: up to max_sub points of one tile (bin[0] x
-// bin[1] x bin[2] fine cells).  A block
-//   1. stages 128 points at a time: coordinates -> grid units in FP64, the
-//      2w tap weights of every axis from the per-tap polynomials (FP32),
-//      values, and the local offset of the first tap inside the padded tile;
-//   2. spread: each wave takes 16 points at once and maps its lanes to
-//      (point, y-tap, z-tap) triples (16 (2w)^2 = 64 w^2 items, so every
-//      lane is busy for every even tap count), loops over the 2w x-taps
-//      and adds into the padded tile with LDS FP64 atomics (split re/im
-//      planes, consecutive z-taps on consecutive banks).  On gfx950 the
-//      LDS FP64 add runs ~23x faster than the LDS FP32 add (measured
-//      7.8 vs 0.33 lane-ops/clk/CU), and the tile accumulates in FP64;
-//   3. flush: the padded tile (bin + 2w per axis) is rounded to FP32 and
-//      added to the HBM grid with global float atomics, skipping zero
-//      cells, periodic wrap.
-// Interpolation loads the padded tile from HBM once, each wave gathers one
-// point at a time (lanes over (y,z) taps, loop over x) and reduces across
-// the wave.
+// 3D work unit = subproblem: up to max_sub points of one tile.
+//
+// Spreading (Type-1) is output-stationary: a 1024-thread workgroup owns the
+// padded tile of a subproblem, lane (py, pz) of the workgroup owns one (y,z)
+// column of it (32 x 32 columns; each wave owns an 8 x 8 block of columns)
+// and keeps the column's bin_x + 2w x-cells in registers.  Points are staged
+// 256 at a time in LDS (coordinates -> grid units in FP64, 2w FP32 tap
+// weights per axis from the per-tap polynomials, value, tap origin); every
+// wave walks the staged points and skips, with one uniform test, those whose
+// (y,z) support misses its column block; otherwise each lane multiplies the
+// point's value by its (y,z) weight and adds the 2w x-taps into its
+// registers (the x origin is uniform, so a switch selects statically indexed
+// registers).  Nothing is accumulated through LDS atomics: on gfx950 the
+// LDS FP32 atomic add measured 0.33 lane-ops/clk/CU, FP64 7.8, against ~64
+// lane-FMAs/clk/CU for register FMAs.  At the end each lane adds its column
+// to the HBM grid with global float atomics (periodic wrap, zeros skipped).
+//
+// Interpolation (Type-2) loads its own, smaller padded tile from HBM into
+// LDS once per subproblem; each wave gathers one point at a time (lanes over
+// (y,z) taps, loop over x) and reduces across the wave.
 #include "internal.h"
 
 namespace anufft {
 namespace {
 
-constexpr int kStage = 128;       // points staged per round (interp, grad)
-constexpr int kSpreadStage = 64;  // points staged per round (spread)
-constexpr int kSpreadThreads = 512;
-constexpr int kInterpThreads = 256;
+constexpr int kStage = 128;        // points staged per round (interp, grad)
+constexpr int kSpreadStage = 256;  // points staged per round (spread)
+constexpr int kSpreadThreads = 1024;
+constexpr int kInterpThreads = 1024;
 
 // Periodic index; padded tiles may exceed the grid by more than one period
 // when the grid is not much larger than the kernel support.
@@ -52,7 +54,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 template <int W>
 struct Staging {
   static constexpr int T = 2 * W;
-  static constexpr int WS = 3 * T + 1; // weights per point (+1 pad)
+  static constexpr int WS = (3 * T + 3) / 4 * 4; // weights per point, 16-byte rows
 };
 
 // Stage one round of points: weights, values, tap origins (3 tasks per point).
@@ -75,11 +77,11 @@ __device__ __forceinline__ void stage_points(const PointArgs &pa, const Geom &g,
     int i0;
     if (DERIV) {
       float dwt[T];
-      i0 = kb_taps_deriv<W>(t, poly, wt, dwt);
+      i0 = kb_taps_deriv<W>(t, cell, poly, wt, dwt);
 #pragma unroll
       for (int i = 0; i < T; ++i) dwts[p * WS + ax * T + i] = dwt[i];
     } else {
-      i0 = kb_taps<W>(t, poly, wt);
+      i0 = kb_taps<W>(t, cell, poly, wt);
     }
 #pragma unroll
     for (int i = 0; i < T; ++i) wts[p * WS + ax * T + i] = wt[i];
@@ -92,87 +94,147 @@ __device__ __forceinline__ void stage_points(const PointArgs &pa, const Geom &g,
   }
 }
 
+// Spreading staging record of one point (floats): x weights zero-padded to
+// the PX register cells of the tile column, y weights, z weights.
+template <int W>
+struct SpreadRow {
+  static constexpr int T = 2 * W;
+  static constexpr int PX = kSpreadBinX + T - 1;   // x cells a tile column holds
+  static constexpr int XW = (PX + 3) / 4 * 4;      // padded x row (16-byte aligned)
+  static constexpr int RS = XW + 2 * T;            // floats per point
+};
+
+// Stage points for spreading: 3 tasks per point (one per axis).  Writes the
+// x row zero-padded at offset cell_x - X0, the y/z weights, the (y,z) origin
+// of the taps inside the padded tile, the value, and 4-bit row / column
+// masks of the 8 x 8 lane blocks the point's (y,z) support touches.
+template <int W, int NT>
+__device__ __forceinline__ void stage_spread(const PointArgs &pa, const Geom &g,
+                                             const KbPoly &poly, const float2 *values,
+                                             int start, int npts, int bx0, int oy, int oz,
+                                             float *rows, float2 *cv, int *lyz, int *rowm,
+                                             int *colm) {
+  constexpr int T = 2 * W;
+  constexpr int PX = SpreadRow<W>::PX;
+  constexpr int XW = SpreadRow<W>::XW;
+  constexpr int RS = SpreadRow<W>::RS;
+  for (int task = threadIdx.x; task < 3 * npts; task += NT) {
+    const int ax = task / npts;
+    const int p = task - ax * npts;
+    const uint32_t j = pa.perm[start + p];
+    const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
+    int cell;
+    const double t = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
+    float wt[T];
+    const int i0 = kb_taps<W>(t, cell, poly, wt);
+    float *r = rows + p * RS;
+    if (ax == 0) {
+      const int lx = cell - bx0;  // in [0, bin_x)
+#pragma unroll
+      for (int i = 0; i < XW; ++i) {
+        const int k = i - lx;
+        float v = 0.f;
+#pragma unroll
+        for (int q = 0; q < T; ++q) v = (k == q) ? wt[q] : v;
+        r[i] = i < PX ? v : 0.f;
+      }
+      cv[p] = values[j];
+    } else {
+      const int l = i0 - (ax == 1 ? oy : oz);  // first tap inside the padded tile
+#pragma unroll
+      for (int i = 0; i < T; ++i) r[XW + (ax - 1) * T + i] = wt[i];
+      lyz[2 * p + ax - 1] = l;
+      const int m = ((1 << ((l + T - 1) / 8 + 1)) - 1) & ~((1 << (l / 8)) - 1);
+      if (ax == 1) rowm[p] = m;
+      else colm[p] = m;
+    }
+  }
+}
+
 template <int W>
 __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
                                                                   const float2 *values,
                                                                   float2 *grid, Geom g,
                                                                   KbPoly poly) {
   constexpr int T = 2 * W;
-  constexpr int WS = Staging<W>::WS;
   constexpr int NT = kSpreadThreads;
-  extern __shared__ double shd[];
-  const int Px = g.pad[0], Py = g.pad[1], Pz = g.pad[2];
-  const int P = Px * Py * Pz;
-  const int PyPz = Py * Pz;
-  double *tre = shd;
-  double *tim = shd + P;
-  float *wts = (float *)(shd + 2 * P);
-  float2 *cv = (float2 *)(wts + ((kSpreadStage * WS + 1) & ~1));
-  int *loc = (int *)(cv + kSpreadStage);
+  constexpr int PX = SpreadRow<W>::PX;
+  constexpr int XW = SpreadRow<W>::XW;
+  constexpr int RS = SpreadRow<W>::RS;
+  extern __shared__ float sh[];
+  float *rows = sh;
+  float2 *cv = (float2 *)(rows + kSpreadStage * RS);
+  int *lyz = (int *)(cv + kSpreadStage);
+  int *rowm = lyz + 2 * kSpreadStage;
+  int *colm = rowm + kSpreadStage;
 
-  for (int i = threadIdx.x; i < 2 * P; i += NT) shd[i] = 0.0;
-  __syncthreads();
-
-  const uint32_t nsub = *pa.nsub;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int kWaves = NT / 64;
+  const int wyb = wave >> 2, wzb = wave & 3;  // this wave's 8 x 8 column block
+  const int py = wyb * 8 + (lane >> 3), pz = wzb * 8 + (lane & 7);
+  const bool col_ok = py < g.pad[1] && pz < g.pad[2];
+  const uint32_t nsub = *pa.nsub;
+
   for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
     const int by = (bin / g.nbin[2]) % g.nbin[1];
     const int bx = bin / (g.nbin[2] * g.nbin[1]);
-    const int ox = bx * g.bin[0] - W, oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
+    const int bx0 = bx * g.bin[0];
+    const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
 
+    float ar[PX], ai[PX];
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+      ar[i] = 0.f;
+      ai[i] = 0.f;
+    }
     for (int c0 = 0; c0 < sub.z; c0 += kSpreadStage) {
       const int npts = min(kSpreadStage, sub.z - c0);
-      stage_points<W, NT, false>(pa, g, poly, values, sub.y + c0, npts, ox, oy, oz, wts,
-                                 nullptr, cv, loc, nullptr);
       __syncthreads();
-      for (int g0 = wave * 16; g0 < npts; g0 += kWaves * 16) {
-        for (int r = 0; r < W * W; ++r) {
-          const int q = lane + 64 * r;
-          const int pl = q / (T * T);
-          const int rem = q - pl * (T * T);
-          const int ty = rem / T, tz = rem - (rem / T) * T;
-          const int p = g0 + pl;
-          if (p < npts) {
-            const float *wp = wts + p * WS;
-            const float wyz = wp[T + ty] * wp[2 * T + tz];
-            const float2 c = cv[p];
-            const float cr = c.x * wyz, ci = c.y * wyz;
-            const int base = (loc[3 * p] * Py + loc[3 * p + 1] + ty) * Pz + loc[3 * p + 2] + tz;
+      stage_spread<W, NT>(pa, g, poly, values, sub.y + c0, npts, bx0, oy, oz, rows, cv, lyz,
+                          rowm, colm);
+      __syncthreads();
+      for (int p0 = 0; p0 < npts; p0 += 64) {
+        const int q = p0 + lane;
+        const bool hit = q < npts && ((rowm[q] >> wyb) & (colm[q] >> wzb) & 1);
+        uint64_t mask = __ballot(hit);
+        while (mask) {
+          const int k = __ffsll((unsigned long long)mask) - 1;
+          mask &= mask - 1;
+          const int p = p0 + k;
+          const float *r = rows + p * RS;
+          int ty = py - lyz[2 * p], tz = pz - lyz[2 * p + 1];
+          const bool act = col_ok && ty >= 0 && ty < T && tz >= 0 && tz < T;
+          ty = min(max(ty, 0), T - 1);
+          tz = min(max(tz, 0), T - 1);
+          const float wyz = act ? r[XW + ty] * r[XW + T + tz] : 0.f;
+          const float2 c = cv[p];
+          const float cr = c.x * wyz, ci = c.y * wyz;
 #pragma unroll
-            for (int tx = 0; tx < T; ++tx) {
-              const float wx = wp[tx];
-              atomicAdd(&tre[base + tx * PyPz], (double)(cr * wx));
-              atomicAdd(&tim[base + tx * PyPz], (double)(ci * wx));
-            }
+          for (int i = 0; i < PX; ++i) {
+            const float wx = r[i];
+            ar[i] = fmaf(cr, wx, ar[i]);
+            ai[i] = fmaf(ci, wx, ai[i]);
           }
         }
       }
-      __syncthreads();
     }
 
-    // flush the padded tile into the grid and clear it
-    for (int i = threadIdx.x; i < P; i += NT) {
-      const double re = tre[i], im = tim[i];
-      if (re != 0.0 || im != 0.0) {
-        tre[i] = 0.0;
-        tim[i] = 0.0;
-        const int pz = i % Pz;
-        const int t2 = i / Pz;
-        const int py = t2 % Py;
-        const int px = t2 / Py;
-        const int gx = wrap(ox + px, g.m[0]);
-        const int gy = wrap(oy + py, g.m[1]);
-        const int gz = wrap(oz + pz, g.m[2]);
-        float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
-        unsafeAtomicAdd(&dst->x, (float)re);
-        unsafeAtomicAdd(&dst->y, (float)im);
+    // add this lane's column to the grid: cell i <-> x = bx0 - W + 1 + i
+    if (col_ok) {
+      const int gy = wrap(oy + py, g.m[1]);
+      const int gz = wrap(oz + pz, g.m[2]);
+#pragma unroll
+      for (int i = 0; i < PX; ++i) {
+        if (ar[i] != 0.f || ai[i] != 0.f) {
+          const int gx = wrap(bx0 - W + 1 + i, g.m[0]);
+          float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
+          unsafeAtomicAdd(&dst->x, ar[i]);
+          unsafeAtomicAdd(&dst->y, ai[i]);
+        }
       }
     }
-    __syncthreads();
   }
 }
 
@@ -360,7 +422,7 @@ __global__ void spread1d_kernel(const float *x, int64_t stride, int64_t n,
     int cell;
     double t = fold_coord(x[j * stride], g.scale[0], m, cell);
     float wt[T];
-    int i0 = kb_taps<W>(t, poly, wt);
+    int i0 = kb_taps<W>(t, cell, poly, wt);
     const float2 c = values[j];
 #pragma unroll
     for (int i = 0; i < T; ++i) {
@@ -382,7 +444,7 @@ __global__ void interp1d_kernel(const float *x, int64_t stride, int64_t n, const
     int cell;
     double t = fold_coord(x[j * stride], g.scale[0], m, cell);
     float wt[T];
-    int i0 = kb_taps<W>(t, poly, wt);
+    int i0 = kb_taps<W>(t, cell, poly, wt);
     float ar = 0.f, ai = 0.f;
 #pragma unroll
     for (int i = 0; i < T; ++i) {
@@ -406,7 +468,7 @@ __global__ void grad1d_kernel(const float *x, int64_t stride, int64_t n, const f
     int cell;
     double t = fold_coord(x[j * stride], g.scale[0], m, cell);
     float wt[T], dwt[T];
-    int i0 = kb_taps_deriv<W>(t, poly, wt, dwt);
+    int i0 = kb_taps_deriv<W>(t, cell, poly, wt, dwt);
     float dr = 0.f, di = 0.f;
 #pragma unroll
     for (int i = 0; i < T; ++i) {
@@ -422,11 +484,9 @@ __global__ void grad1d_kernel(const float *x, int64_t stride, int64_t n, const f
 }
 
 template <int W>
-size_t spread_lds(const Geom &g) {
-  const size_t P = (size_t)g.pad[0] * g.pad[1] * g.pad[2];
-  const size_t ws = ((size_t)kSpreadStage * Staging<W>::WS + 1) & ~(size_t)1;
-  return 2 * P * sizeof(double) + ws * sizeof(float) + kSpreadStage * sizeof(float2) +
-         3 * kSpreadStage * sizeof(int);
+size_t spread_lds(const Geom &) {
+  return (size_t)kSpreadStage * SpreadRow<W>::RS * sizeof(float) + kSpreadStage * sizeof(float2) +
+         4 * kSpreadStage * sizeof(int);
 }
 
 template <int W>

@@ -28,6 +28,8 @@ def _meta_dict(m):
         "compat_reference_type2_3d_scaling": bool(m.compat_reference_type2_3d_scaling),
         "bin_size": tuple(m.bin_size),
         "max_subproblem": m.max_subproblem,
+        "interp_bin_size": tuple(m.interp_bin_size),
+        "interp_max_subproblem": m.interp_max_subproblem,
         "poly_degree": m.poly_degree,
         "poly_max_err": m.poly_max_err,
     }

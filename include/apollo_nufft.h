@@ -66,8 +66,8 @@ typedef struct anufft_opts {
   int32_t grid_policy;                       /* anufft_grid_policy */
   int32_t compat_reference_type2_3d_scaling; /* 1: 3D T2 returns direct/(mx my mz) like the reference */
   int32_t device_id;                         /* -1: current device */
-  int32_t bin_size[3];                       /* spreading tile in fine cells; 0 = automatic */
-  int32_t max_subproblem;                    /* points per spreading subproblem; 0 = automatic */
+  int32_t bin_size[3];                       /* interpolation tile in cells; 0 = automatic */
+  int32_t max_subproblem;                    /* points per subproblem; 0 = automatic */
 } anufft_opts;
 
 typedef struct anufft_meta {
@@ -81,8 +81,10 @@ typedef struct anufft_meta {
   double i0_beta;
   int32_t grid_policy;
   int32_t compat_reference_type2_3d_scaling;
-  int32_t bin_size[3];
-  int32_t max_subproblem;
+  int32_t bin_size[3];        /* spreading tile (cells) */
+  int32_t max_subproblem;     /* points per spreading subproblem */
+  int32_t interp_bin_size[3]; /* interpolation / gradient tile (cells) */
+  int32_t interp_max_subproblem;
   int32_t poly_degree;  /* degree of the per-tap kernel polynomials */
   double poly_max_err;  /* max |phi - fit| on [-w, w] measured at plan creation */
 } anufft_meta;
