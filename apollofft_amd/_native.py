@@ -44,7 +44,7 @@ EXPORTS = [
     "anufft_timing_enable", "anufft_timing_collect",
 ]
 
-STAGES = ["fill", "bin", "spread", "fft", "interp", "grad"]
+STAGES = ["fill", "bin", "spread", "fft", "interp", "grad", "prep"]
 
 
 class NufftError(RuntimeError):

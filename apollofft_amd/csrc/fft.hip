@@ -122,7 +122,9 @@ __device__ __forceinline__ void stage(const float2 *src, float2 *dst, int L, int
   }
 }
 
-// Any radix (primes above 5): one thread per output element.
+// Any radix (primes above 5): one thread per output element.  With
+// W_R = W_L^(L/R), the twiddle of input r is W_L^(r (k twstep + u L/R)), so
+// its exponent advances by a fixed step modulo L.
 __device__ void stage_generic(const float2 *src, float2 *dst, int L, int Ns, int R,
                               const float2 *tw, int lines) {
   const int nbf = L / R;
@@ -134,11 +136,14 @@ __device__ void stage_generic(const float2 *src, float2 *dst, int L, int Ns, int
     int u = (o / Ns) % R;
     int jq = o / (Ns * R);
     int j = jq * Ns + k;
-    const float2 *s = src + li * L;
-    float2 acc = make_float2(0.f, 0.f);
-    for (int r = 0; r < R; ++r) {
-      int e = (k * r * twstep + ((r * u) % R) * nbf) % L;
-      acc = cadd(acc, cmul(s[j + r * nbf], tw[e]));
+    const float2 *s = src + li * L + j;
+    const int step = (k * twstep + u * nbf) % L;
+    float2 acc = s[0];
+    int e = 0;
+    for (int r = 1; r < R; ++r) {
+      e += step;
+      if (e >= L) e -= L;
+      acc = cadd(acc, cmul(s[r * nbf], tw[e]));
     }
     dst[li * L + o] = acc;
   }
@@ -154,6 +159,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   const int L = p.L, lb = p.lb;
   float2 *buf0 = sh;
   float2 *buf1 = sh + (size_t)lb * L;
+  float2 *tw = sh + 2 * (size_t)lb * L;  // twiddle table of this length
+  for (int i = threadIdx.x; i < L; i += kFftThreads) tw[i] = p.tw[i];
   const int64_t l0 = (int64_t)blockIdx.x * lb;
   const int nl = (int)((p.nlines - l0) < lb ? (p.nlines - l0) : lb);
 
@@ -194,12 +201,12 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     __syncthreads();
     const int R = p.fac[f];
     switch (R) {
-    case 2: stage<2>(src, dst, L, Ns, p.tw, nl); break;
-    case 3: stage<3>(src, dst, L, Ns, p.tw, nl); break;
-    case 4: stage<4>(src, dst, L, Ns, p.tw, nl); break;
-    case 5: stage<5>(src, dst, L, Ns, p.tw, nl); break;
-    case 8: stage<8>(src, dst, L, Ns, p.tw, nl); break;
-    default: stage_generic(src, dst, L, Ns, R, p.tw, nl); break;
+    case 2: stage<2>(src, dst, L, Ns, tw, nl); break;
+    case 3: stage<3>(src, dst, L, Ns, tw, nl); break;
+    case 4: stage<4>(src, dst, L, Ns, tw, nl); break;
+    case 5: stage<5>(src, dst, L, Ns, tw, nl); break;
+    case 8: stage<8>(src, dst, L, Ns, tw, nl); break;
+    default: stage_generic(src, dst, L, Ns, R, tw, nl); break;
     }
     float2 *t = src;
     src = dst;
@@ -241,7 +248,7 @@ __global__ void fill_kernel(float2 *p, int64_t n) {
 hipError_t launch_fft_pass(const FftPass &p, hipStream_t s) {
   if (p.nlines <= 0) return hipSuccess;
   int64_t blocks = (p.nlines + p.lb - 1) / p.lb;
-  size_t lds = 2 * (size_t)p.lb * p.L * sizeof(float2);
+  size_t lds = (2 * (size_t)p.lb + 1) * p.L * sizeof(float2);
   hipLaunchKernelGGL(fft_pass_kernel, dim3((unsigned)blocks), dim3(kFftThreads), lds, s, p);
   return hipGetLastError();
 }

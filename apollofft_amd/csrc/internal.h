@@ -88,6 +88,22 @@ struct PointArgs {
   const uint32_t *nsub;      // device count of subproblems
 };
 
+// One problem of a batched spreading launch.
+struct SpreadProblem {
+  PointArgs pa;
+  const float *rec;    // per-point spreading records (launch_spread_xrec)
+  const uint8_t *xo;   // per-point x offsets in the tile
+  float2 *grid;
+  Geom g;
+};
+
+constexpr int kBatchChunk = 16;  // descriptors per table-fill launch (kernarg size)
+struct BatchChunk {
+  int first;
+  int count;
+  SpreadProblem probs[kBatchChunk];
+};
+
 // ---- launchers (defined in the .hip files) ---------------------------------
 hipError_t launch_fill(float2 *p, int64_t n, hipStream_t s);
 
@@ -96,8 +112,17 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
 
 hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
 
-hipError_t launch_spread3d(int w, const PointArgs &pa, const float2 *values, float2 *grid,
-                           const Geom &g, const KbPoly &poly, int num_blocks, hipStream_t s);
+// Spreading records: spread_rec_floats(w) floats and one x-offset byte per
+// sorted point, filled by launch_spread_xrec and read by launch_spread3d.
+int spread_rec_floats(int w);
+hipError_t launch_spread_xrec(int w, const PointArgs &pa, const float2 *values, const Geom &g,
+                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
+hipError_t launch_spread3d(int w, const PointArgs &pa, const float *rec, const uint8_t *xo,
+                           float2 *grid, const Geom &g, const KbPoly &poly, int num_blocks,
+                           hipStream_t s);
+hipError_t launch_spread3d_batched(int w, const SpreadProblem *host_probs, int nprob,
+                                   SpreadProblem *dev_table, uint32_t *dev_prefix,
+                                   const KbPoly &poly, int num_blocks, hipStream_t s);
 hipError_t launch_interp3d(int w, const PointArgs &pa, const float2 *grid, float2 *out,
                            const Geom &g, const KbPoly &poly, int num_blocks, hipStream_t s);
 hipError_t launch_grad3d(int w, const PointArgs &pa, const float2 *grid, const float2 *u,

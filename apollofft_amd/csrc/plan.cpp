@@ -181,7 +181,7 @@ struct StageTimer {
 
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
-         sub_ofs = 0, subs = 0, grid = 0, fftbuf = 0, total = 0;
+         sub_ofs = 0, subs = 0, rec = 0, xo = 0, grid = 0, fftbuf = 0, total = 0;
   int nchunks = 0;
 };
 
@@ -259,6 +259,8 @@ Layout layout(const anufft_plan *p, int64_t M) {
     int64_t nsub = std::max<int64_t>(p->bs.nbins + (M + p->bs.max_sub - 1) / p->bs.max_sub,
                                      p->bi.nbins + (M + p->bi.max_sub - 1) / p->bi.max_sub) + 1;
     l.subs = take(16 * (size_t)nsub);
+    l.rec = take(sizeof(float) * (size_t)spread_rec_floats(p->w) * Mp);
+    l.xo = take(Mp);
     l.grid = take(8 * (size_t)p->cells());
     l.fftbuf = take(8 * (size_t)(p->m[0] * p->m[1] * p->n[2]));
   } else {
@@ -529,6 +531,19 @@ struct Exec {
     return e;
   }
   float2 *grid() const { return (float2 *)(ws + lay.grid); }
+  float *rec() const { return (float *)(ws + lay.rec); }
+  uint8_t *xo() const { return (uint8_t *)(ws + lay.xo); }
+  // Type-1 spreading of sorted points: records, then the tile kernel.
+  hipError_t spread3(const PointArgs &pa, const float2 *values, float2 *grid) const {
+    hipError_t e = timed(ANUFFT_STAGE_PREP, [&] {
+      return launch_spread_xrec(p->w, pa, values, p->bs.g, p->poly, rec(), xo(), s);
+    });
+    if (e != hipSuccess) return e;
+    return timed(ANUFFT_STAGE_SPREAD, [&] {
+      return launch_spread3d(p->w, pa, rec(), xo(), grid, p->bs.g, p->poly,
+                             blocks(p->bs, pa.num_points, 2), s);
+    });
+  }
   float2 *fftbuf() const { return (float2 *)(ws + lay.fftbuf); }
   SortBuffers sort_buffers(const anufft_plan::Binning &b) const {
     SortBuffers sb{};
@@ -774,10 +789,7 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
       PointArgs pa = ex.point_args(p->bs, M, coords, stride);
       e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bs.g, 3, ex.sort_buffers(p->bs), ex.s); });
       if (e != hipSuccess) return hip_fail(e, "binning");
-      e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
-        return launch_spread3d(p->w, pa, (const float2 *)values, ex.grid(), p->bs.g, p->poly,
-                               ex.blocks(p->bs, M, 2), ex.s);
-      });
+      e = ex.spread3(pa, (const float2 *)values, ex.grid());
     } else {
       e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
         return launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(),
@@ -939,8 +951,7 @@ anufft_status anufft_debug_spread(const anufft_plan *p, int64_t M, const float *
     PointArgs pa = ex.point_args(p->bs, M, coords, stride);
     e = sort_points(pa, p->bs.g, 3, ex.sort_buffers(p->bs), ex.s);
     if (e != hipSuccess) return hip_fail(e, "binning");
-    e = launch_spread3d(p->w, pa, (const float2 *)values, (float2 *)grid, p->bs.g, p->poly,
-                        ex.blocks(p->bs, M, 2), ex.s);
+    e = ex.spread3(pa, (const float2 *)values, (float2 *)grid);
   } else {
     e = launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, (float2 *)grid,
                         p->bs.g, p->poly, ex.s);
@@ -956,12 +967,19 @@ anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32
   for (int i = 0; i < nprob; ++i) {
     if (!plans[i]) return fail(ANUFFT_INVALID_ARG, "plan %d is NULL", i);
     if (plans[i]->dim != 3) return fail(ANUFFT_INVALID_ARG, "batched execution takes 3D plans");
+    if (max_points[i] < 0) return fail(ANUFFT_INVALID_ARG, "max_points[%d] < 0", i);
     tot += align_up(layout(plans[i], max_points[i]).total);
   }
+  tot += align_up(sizeof(SpreadProblem) * (size_t)std::max(nprob, 1));
+  tot += align_up(sizeof(uint32_t) * (size_t)(nprob + 1));
   *bytes = tot;
   return ANUFFT_OK;
 }
 
+// Batched Type-1: per-problem binning and FFT passes on one stream, and the
+// spreading of every problem in one persistent launch (problems must share
+// kernel_width and sigma, hence the kernel polynomials; otherwise the
+// problems run one after another).
 anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int32_t nprob,
                                            const int64_t *num_points, const float *const *coords,
                                            int64_t stride, const float *const *values,
@@ -972,16 +990,92 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   if (st != ANUFFT_OK) return st;
   if (!ws || ws_bytes < need)
     return fail(ANUFFT_CAPACITY_EXCEEDED, "workspace of %zu bytes is smaller than the %zu required", ws_bytes, need);
+  if (nprob == 0) return ANUFFT_OK;
   if (!coords || !values || !out) return fail(ANUFFT_INVALID_ARG, "NULL argument");
-  size_t off = 0;
+  bool fused = true;
   for (int i = 0; i < nprob; ++i) {
-    const float *c3[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
-    size_t sz = align_up(layout(plans[i], num_points[i]).total);
-    st = anufft_execute_type1(plans[i], num_points[i], c3, stride, values[i], out[i],
-                              (char *)ws + off, sz, stream);
-    if (st != ANUFFT_OK) return st;
-    off += sz;
+    if (plans[i]->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for plan %d", i);
+    if (plans[i]->device != plans[0]->device) return fail(ANUFFT_INVALID_ARG, "plans live on different devices");
+    if (!out[i] || (num_points[i] > 0 && (!values[i] || !coords[3 * i] || !coords[3 * i + 1] || !coords[3 * i + 2])))
+      return fail(ANUFFT_INVALID_ARG, "NULL pointer in problem %d", i);
+    if (num_points[i] > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points[%d] exceeds u32", i);
+    if (plans[i]->w != plans[0]->w || plans[i]->sigma != plans[0]->sigma) fused = false;
   }
+  const anufft_plan *p0 = plans[0];
+  DeviceGuard guard(p0->device);
+  hipStream_t s = (hipStream_t)stream;
+  if (!fused) {
+    size_t off = 0;
+    for (int i = 0; i < nprob; ++i) {
+      const float *c3[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+      size_t sz = align_up(layout(plans[i], num_points[i]).total);
+      st = anufft_execute_type1(plans[i], num_points[i], c3, stride, values[i], out[i],
+                                (char *)ws + off, sz, stream);
+      if (st != ANUFFT_OK) return st;
+      off += sz;
+    }
+    return ANUFFT_OK;
+  }
+  std::vector<Exec> ex;
+  std::vector<SpreadProblem> probs(nprob);
+  ex.reserve(nprob);
+  size_t off = 0;
+  int64_t sub_upper = 0;
+  for (int i = 0; i < nprob; ++i) {
+    Layout lay = layout(plans[i], num_points[i]);
+    ex.push_back(Exec{plans[i], s, (char *)ws + off, lay});
+    off += align_up(lay.total);
+    const float *c3[3] = {coords[3 * i], coords[3 * i + 1], coords[3 * i + 2]};
+    probs[i].pa = ex[i].point_args(plans[i]->bs, num_points[i], c3, stride);
+    probs[i].rec = ex[i].rec();
+    probs[i].xo = ex[i].xo();
+    probs[i].grid = ex[i].grid();
+    probs[i].g = plans[i]->bs.g;
+    sub_upper += plans[i]->bs.nbins + (num_points[i] + plans[i]->bs.max_sub - 1) / plans[i]->bs.max_sub;
+  }
+  SpreadProblem *table = (SpreadProblem *)((char *)ws + off);
+  off += align_up(sizeof(SpreadProblem) * (size_t)nprob);
+  uint32_t *prefix = (uint32_t *)((char *)ws + off);
+  hipError_t e;
+  const Exec &e0 = ex[0];
+  e = e0.timed(ANUFFT_STAGE_FILL, [&] {
+    for (int i = 0; i < nprob; ++i) {
+      hipError_t r = launch_fill(ex[i].grid(), plans[i]->cells(), s);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  });
+  if (e != hipSuccess) return hip_fail(e, "fill");
+  e = e0.timed(ANUFFT_STAGE_BIN, [&] {
+    for (int i = 0; i < nprob; ++i) {
+      hipError_t r = sort_points(probs[i].pa, plans[i]->bs.g, 3, ex[i].sort_buffers(plans[i]->bs), s);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  });
+  if (e != hipSuccess) return hip_fail(e, "binning");
+  e = e0.timed(ANUFFT_STAGE_PREP, [&] {
+    for (int i = 0; i < nprob; ++i) {
+      hipError_t r = launch_spread_xrec(p0->w, probs[i].pa, (const float2 *)values[i],
+                                        plans[i]->bs.g, plans[i]->poly, ex[i].rec(), ex[i].xo(), s);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  });
+  if (e != hipSuccess) return hip_fail(e, "spread records");
+  const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(sub_upper, 2 * (int64_t)p0->cus));
+  e = e0.timed(ANUFFT_STAGE_SPREAD, [&] {
+    return launch_spread3d_batched(p0->w, probs.data(), nprob, table, prefix, p0->poly, nblk, s);
+  });
+  if (e != hipSuccess) return hip_fail(e, "spread");
+  e = e0.timed(ANUFFT_STAGE_FFT, [&] {
+    for (int i = 0; i < nprob; ++i) {
+      hipError_t r = ex[i].grid_to_modes((float2 *)out[i]);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  });
+  if (e != hipSuccess) return hip_fail(e, "fft");
   return ANUFFT_OK;
 }
 

@@ -45,10 +45,21 @@ __device__ __forceinline__ int wrap(int v, int m) {
   return v;
 }
 
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
+}
+
+// Sum over the 64 lanes of a wave with DPP (quad swaps, row mirrors, row
+// broadcasts) instead of LDS permutes; the total is read from lane 63.
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp<0xB1>(v);        // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);        // quad_perm [2,3,0,1]
+  v += dpp<0x141>(v);       // row_half_mirror
+  v += dpp<0x140>(v);       // row_mirror: every lane holds its row's sum
+  v += dpp<0x142, 0xa>(v);  // row_bcast15 into rows 1 and 3
+  v += dpp<0x143, 0xc>(v);  // row_bcast31 into rows 2 and 3
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
 template <int W>
@@ -94,148 +105,237 @@ __device__ __forceinline__ void stage_points(const PointArgs &pa, const Geom &g,
   }
 }
 
-// Spreading staging record of one point (floats): x weights zero-padded to
-// the PX register cells of the tile column, y weights, z weights.
+// Per-point spreading record, in sorted order: the value (re, im) and the 2w
+// x-weights, zero-padded to a multiple of 4 floats.  The spreading kernel
+// reads a record with wave-uniform scalar loads, so the x-weights reach the
+// FMAs as scalar operands instead of 64-lane LDS broadcasts.
 template <int W>
-struct SpreadRow {
+struct XRec {
   static constexpr int T = 2 * W;
-  static constexpr int PX = kSpreadBinX + T - 1;   // x cells a tile column holds
-  static constexpr int XW = (PX + 3) / 4 * 4;      // padded x row (16-byte aligned)
-  static constexpr int RS = XW + 2 * T;            // floats per point
+  static constexpr int RW = (T + 2 + 3) / 4 * 4;  // floats per record
 };
 
-// Stage points for spreading: 3 tasks per point (one per axis).  Writes the
-// x row zero-padded at offset cell_x - X0, the y/z weights, the (y,z) origin
-// of the taps inside the padded tile, the value, and 4-bit row / column
-// masks of the 8 x 8 lane blocks the point's (y,z) support touches.
+// Fill the records and the x offset cell - bin_x floor(cell / bin_x) of the
+// first tap's register cell, for every sorted position.
+template <int W>
+__global__ __launch_bounds__(256) void spread_xrec_kernel(PointArgs pa, const float2 *values,
+                                                           Geom g, KbPoly poly, float4 *rec,
+                                                           uint8_t *xo) {
+  constexpr int T = 2 * W;
+  constexpr int RW = XRec<W>::RW;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < pa.num_points; s += st) {
+    const uint32_t j = pa.perm[s];
+    int cell;
+    const double t = fold_coord(pa.x[(int64_t)j * pa.stride], g.scale[0], g.m[0], cell);
+    float r[RW];
+    kb_taps<W>(t, cell, poly, r + 2);
+    const float2 c = values[j];
+    r[0] = c.x;
+    r[1] = c.y;
+#pragma unroll
+    for (int i = T + 2; i < RW; ++i) r[i] = 0.f;
+    float4 *d = rec + s * (RW / 4);
+#pragma unroll
+    for (int q = 0; q < RW / 4; ++q) d[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+    xo[s] = (uint8_t)(cell - (cell / g.bin[0]) * g.bin[0]);
+  }
+}
+
+// LDS staging of the y and z weights of one point, zero-padded to the 32
+// column positions of the padded tile, so a lane reads its weight at its own
+// column index.
+constexpr int kSpreadRow = 2 * kSpreadCols;  // floats per staged point
+
+// Stage points for spreading: 2 tasks per point (y and z).  Each zero-fills
+// its row, stores the 2w weights at the taps' column, and writes a 4-bit
+// mask of the 8-column lane blocks the point's support touches; the y task
+// also copies the point's x offset.
 template <int W, int NT>
 __device__ __forceinline__ void stage_spread(const PointArgs &pa, const Geom &g,
-                                             const KbPoly &poly, const float2 *values,
-                                             int start, int npts, int bx0, int oy, int oz,
-                                             float *rows, float2 *cv, int *lyz, int *rowm,
-                                             int *colm) {
+                                             const float *pc, const uint8_t *xo, int start,
+                                             int npts, int oy, int oz, float *rows, int *rowm,
+                                             int *colm, int *xos) {
   constexpr int T = 2 * W;
-  constexpr int PX = SpreadRow<W>::PX;
-  constexpr int XW = SpreadRow<W>::XW;
-  constexpr int RS = SpreadRow<W>::RS;
-  for (int task = threadIdx.x; task < 3 * npts; task += NT) {
-    const int ax = task / npts;
-    const int p = task - ax * npts;
+  for (int task = threadIdx.x; task < 2 * npts; task += NT) {
+    const int ax = task < npts ? 1 : 2;
+    const int p = task - (ax - 1) * npts;
     const uint32_t j = pa.perm[start + p];
-    const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
+    const float *cp = ax == 1 ? pa.y : pa.z;
     int cell;
     const double t = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
-    float wt[T];
-    const int i0 = kb_taps<W>(t, cell, poly, wt);
-    float *r = rows + p * RS;
-    if (ax == 0) {
-      const int lx = cell - bx0;  // in [0, bin_x)
+    const int l = cell - W + 1 - (ax == 1 ? oy : oz);  // first tap's column, in [1, bin]
+    float *r = rows + p * kSpreadRow + (ax - 1) * kSpreadCols;
 #pragma unroll
-      for (int i = 0; i < XW; ++i) {
-        const int k = i - lx;
-        float v = 0.f;
+    for (int i = 0; i < kSpreadCols; i += 4) *(float4 *)(r + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+    // the taps of kb_taps, stored one by one, with the polynomial
+    // coefficients read from the workgroup's LDS copy pc[d][i]
+    const float z = (float)(1.0 - 2.0 * (t - (double)cell));
+#pragma unroll 1
+    for (int i = 0; i < T; ++i) {
+      float v = pc[(kPolyCoefs - 1) * kMaxTaps + i];
 #pragma unroll
-        for (int q = 0; q < T; ++q) v = (k == q) ? wt[q] : v;
-        r[i] = i < PX ? v : 0.f;
-      }
-      cv[p] = values[j];
+      for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, pc[d * kMaxTaps + i]);
+      if (i == 0 && z <= -1.0f) v = 0.0f;
+      if (i == T - 1 && z >= 1.0f) v = 0.0f;
+      r[l + i] = v;
+    }
+    const int m = ((1 << ((l + T - 1) / 8 + 1)) - 1) & ~((1 << (l / 8)) - 1);
+    if (ax == 1) {
+      rowm[p] = m;
+      xos[p] = xo[start + p];
     } else {
-      const int l = i0 - (ax == 1 ? oy : oz);  // first tap inside the padded tile
+      colm[p] = m;
+    }
+  }
+}
+
+typedef const float __attribute__((address_space(4))) *ConstFloatPtr;
+
+// One spreading subproblem (see the file comment): stage, accumulate the
+// lane's column in registers, flush it to the grid.
+// LDS of the spreading kernels: staged rows, masks and x offsets, then the
+// polynomial coefficients (copied once per workgroup by spread_load_poly).
+constexpr int kSpreadPolyOfs = kSpreadStage * (kSpreadRow + 3);  // floats
+
+__device__ __forceinline__ void spread_load_poly(const KbPoly &poly, float *sh) {
+  float *pc = sh + kSpreadPolyOfs;
+  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += blockDim.x)
+    pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
+}
+
+template <int W>
+__device__ __forceinline__ void spread_subproblem(const PointArgs &pa, const float *rec,
+                                                  const uint8_t *xo, float2 *grid, const Geom &g,
+                                                  int4 sub, float *sh) {
+  constexpr int NT = kSpreadThreads;
+  constexpr int T = 2 * W;
+  constexpr int PX = kSpreadBinX + T - 1;  // x cells of a tile column
+  constexpr int RW = XRec<W>::RW;
+  float *rows = sh;
+  int *rowm = (int *)(rows + kSpreadStage * kSpreadRow);
+  int *colm = rowm + kSpreadStage;
+  int *xos = colm + kSpreadStage;
+  const float *pc = sh + kSpreadPolyOfs;
+
+  // Wave -> 8 x 8 column block.  Waves are dealt to the 4 SIMDs round-robin;
+  // the z block is rotated by the y block so each SIMD holds one block of
+  // every row and column of blocks (central blocks see the most points).
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wyb = wave >> 2, wzb = (wave + wyb) & 3;
+  const int py = wyb * 8 + (lane >> 3), pz = wzb * 8 + (lane & 7);
+  const bool col_ok = py < g.pad[1] && pz < g.pad[2];
+
+  const int bin = sub.x;
+  const int bz = bin % g.nbin[2];
+  const int by = (bin / g.nbin[2]) % g.nbin[1];
+  const int bx = bin / (g.nbin[2] * g.nbin[1]);
+  const int bx0 = bx * g.bin[0];
+  const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
+
+  float ar[PX], ai[PX];
 #pragma unroll
-      for (int i = 0; i < T; ++i) r[XW + (ax - 1) * T + i] = wt[i];
-      lyz[2 * p + ax - 1] = l;
-      const int m = ((1 << ((l + T - 1) / 8 + 1)) - 1) & ~((1 << (l / 8)) - 1);
-      if (ax == 1) rowm[p] = m;
-      else colm[p] = m;
+  for (int i = 0; i < PX; ++i) {
+    ar[i] = 0.f;
+    ai[i] = 0.f;
+  }
+  for (int c0 = 0; c0 < sub.z; c0 += kSpreadStage) {
+    const int npts = min(kSpreadStage, sub.z - c0);
+    __syncthreads();
+    stage_spread<W, NT>(pa, g, pc, xo, sub.y + c0, npts, oy, oz, rows, rowm, colm, xos);
+    __syncthreads();
+    const ConstFloatPtr recs = (ConstFloatPtr)(rec + (int64_t)(sub.y + c0) * RW);
+    for (int p0 = 0; p0 < npts; p0 += 64) {
+      const int q = p0 + lane;
+      const bool hit = q < npts && ((rowm[q] >> wyb) & (colm[q] >> wzb) & 1);
+      const int o = q < npts ? xos[q] : 0;
+      // one pass per x offset, so the register cells it adds to are static
+#pragma unroll
+      for (int oo = 0; oo < kSpreadBinX; ++oo) {
+        uint64_t mask = __ballot(hit && o == oo);
+        while (mask) {
+          const int k = __ffsll((unsigned long long)mask) - 1;
+          mask &= mask - 1;
+          const int p = p0 + k;
+          const float *r = rows + p * kSpreadRow;
+          const float wyz = r[py] * r[kSpreadCols + pz];
+          const ConstFloatPtr xr = recs + p * RW;
+          const float cr = xr[0] * wyz, ci = xr[1] * wyz;
+#pragma unroll
+          for (int i = 0; i < T; ++i) {
+            const float wx = xr[2 + i];
+            ar[oo + i] = fmaf(cr, wx, ar[oo + i]);
+            ai[oo + i] = fmaf(ci, wx, ai[oo + i]);
+          }
+        }
+      }
+    }
+  }
+
+  // add this lane's column to the grid: cell i <-> x = bx0 - W + 1 + i
+  if (col_ok) {
+    const int gy = wrap(oy + py, g.m[1]);
+    const int gz = wrap(oz + pz, g.m[2]);
+#pragma unroll
+    for (int i = 0; i < PX; ++i) {
+      if (ar[i] != 0.f || ai[i] != 0.f) {
+        const int gx = wrap(bx0 - W + 1 + i, g.m[0]);
+        float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
+        unsafeAtomicAdd(&dst->x, ar[i]);
+        unsafeAtomicAdd(&dst->y, ai[i]);
+      }
     }
   }
 }
 
 template <int W>
-__global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa,
-                                                                  const float2 *values,
-                                                                  float2 *grid, Geom g,
-                                                                  KbPoly poly) {
-  constexpr int T = 2 * W;
-  constexpr int NT = kSpreadThreads;
-  constexpr int PX = SpreadRow<W>::PX;
-  constexpr int XW = SpreadRow<W>::XW;
-  constexpr int RS = SpreadRow<W>::RS;
+__global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa, const float *rec,
+                                                                  const uint8_t *xo, float2 *grid,
+                                                                  Geom g, KbPoly poly) {
   extern __shared__ float sh[];
-  float *rows = sh;
-  float2 *cv = (float2 *)(rows + kSpreadStage * RS);
-  int *lyz = (int *)(cv + kSpreadStage);
-  int *rowm = lyz + 2 * kSpreadStage;
-  int *colm = rowm + kSpreadStage;
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int wyb = wave >> 2, wzb = wave & 3;  // this wave's 8 x 8 column block
-  const int py = wyb * 8 + (lane >> 3), pz = wzb * 8 + (lane & 7);
-  const bool col_ok = py < g.pad[1] && pz < g.pad[2];
+  spread_load_poly(poly, sh);  // ordered before use by the staging barrier
   const uint32_t nsub = *pa.nsub;
+  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x)
+    spread_subproblem<W>(pa, rec, xo, grid, g, pa.subs[sidx], sh);
+}
 
-  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x) {
-    const int4 sub = pa.subs[sidx];
-    const int bin = sub.x;
-    const int bz = bin % g.nbin[2];
-    const int by = (bin / g.nbin[2]) % g.nbin[1];
-    const int bx = bin / (g.nbin[2] * g.nbin[1]);
-    const int bx0 = bx * g.bin[0];
-    const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
-
-    float ar[PX], ai[PX];
-#pragma unroll
-    for (int i = 0; i < PX; ++i) {
-      ar[i] = 0.f;
-      ai[i] = 0.f;
+// Batched spreading: one persistent launch over the subproblems of all
+// problems.  prefix[p] is the first global subproblem index of problem p
+// (prefix[nprob] = total); the problem of a global index is found by a
+// uniform binary search and its descriptor read with scalar loads.
+template <int W>
+__global__ __launch_bounds__(kSpreadThreads) void spread3d_batched_kernel(
+    const SpreadProblem *probs, const uint32_t *prefix, int nprob, KbPoly poly) {
+  extern __shared__ float sh[];
+  spread_load_poly(poly, sh);  // ordered before use by the staging barrier
+  const uint32_t total = prefix[nprob];
+  for (uint32_t gidx = blockIdx.x; gidx < total; gidx += gridDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= gidx) lo = mid;
+      else hi = mid - 1;
     }
-    for (int c0 = 0; c0 < sub.z; c0 += kSpreadStage) {
-      const int npts = min(kSpreadStage, sub.z - c0);
-      __syncthreads();
-      stage_spread<W, NT>(pa, g, poly, values, sub.y + c0, npts, bx0, oy, oz, rows, cv, lyz,
-                          rowm, colm);
-      __syncthreads();
-      for (int p0 = 0; p0 < npts; p0 += 64) {
-        const int q = p0 + lane;
-        const bool hit = q < npts && ((rowm[q] >> wyb) & (colm[q] >> wzb) & 1);
-        uint64_t mask = __ballot(hit);
-        while (mask) {
-          const int k = __ffsll((unsigned long long)mask) - 1;
-          mask &= mask - 1;
-          const int p = p0 + k;
-          const float *r = rows + p * RS;
-          int ty = py - lyz[2 * p], tz = pz - lyz[2 * p + 1];
-          const bool act = col_ok && ty >= 0 && ty < T && tz >= 0 && tz < T;
-          ty = min(max(ty, 0), T - 1);
-          tz = min(max(tz, 0), T - 1);
-          const float wyz = act ? r[XW + ty] * r[XW + T + tz] : 0.f;
-          const float2 c = cv[p];
-          const float cr = c.x * wyz, ci = c.y * wyz;
-#pragma unroll
-          for (int i = 0; i < PX; ++i) {
-            const float wx = r[i];
-            ar[i] = fmaf(cr, wx, ar[i]);
-            ai[i] = fmaf(ci, wx, ai[i]);
-          }
-        }
-      }
-    }
-
-    // add this lane's column to the grid: cell i <-> x = bx0 - W + 1 + i
-    if (col_ok) {
-      const int gy = wrap(oy + py, g.m[1]);
-      const int gz = wrap(oz + pz, g.m[2]);
-#pragma unroll
-      for (int i = 0; i < PX; ++i) {
-        if (ar[i] != 0.f || ai[i] != 0.f) {
-          const int gx = wrap(bx0 - W + 1 + i, g.m[0]);
-          float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
-          unsafeAtomicAdd(&dst->x, ar[i]);
-          unsafeAtomicAdd(&dst->y, ai[i]);
-        }
-      }
-    }
+    const SpreadProblem &pr = probs[lo];
+    spread_subproblem<W>(pr.pa, pr.rec, pr.xo, pr.grid, pr.g, pr.pa.subs[gidx - prefix[lo]], sh);
   }
+}
+
+// Write descriptors passed by value into the device table.
+__global__ void batch_table_kernel(BatchChunk chunk, SpreadProblem *table) {
+  const int i = threadIdx.x;
+  if (i < chunk.count) table[chunk.first + i] = chunk.probs[i];
+}
+
+// prefix[p] = sum of the subproblem counts of problems < p.
+__global__ void batch_prefix_kernel(const SpreadProblem *probs, int nprob, uint32_t *prefix) {
+  if (threadIdx.x != 0) return;
+  uint32_t acc = 0;
+  for (int p = 0; p < nprob; ++p) {
+    prefix[p] = acc;
+    acc += *probs[p].pa.nsub;
+  }
+  prefix[nprob] = acc;
 }
 
 // Load the padded tile of subproblem origin (ox,oy,oz) from the grid.
@@ -485,8 +585,7 @@ __global__ void grad1d_kernel(const float *x, int64_t stride, int64_t n, const f
 
 template <int W>
 size_t spread_lds(const Geom &) {
-  return (size_t)kSpreadStage * SpreadRow<W>::RS * sizeof(float) + kSpreadStage * sizeof(float2) +
-         4 * kSpreadStage * sizeof(int);
+  return ((size_t)kSpreadPolyOfs + kPolyCoefs * kMaxTaps) * sizeof(float);
 }
 
 template <int W>
@@ -511,15 +610,43 @@ hipError_t dispatch_w(int w, A &&...args) {
 }
 
 template <int W>
+struct SpreadXrec {
+  static hipError_t run(const PointArgs &pa, const float2 *values, const Geom &g,
+                        const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s) {
+    if (pa.num_points <= 0) return hipSuccess;
+    int64_t blocks = (pa.num_points + 255) / 256;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(spread_xrec_kernel<W>, dim3((unsigned)blocks), dim3(256), 0, s, pa, values,
+                       g, poly, (float4 *)rec, xo);
+    return hipGetLastError();
+  }
+};
+
+template <int W>
 struct Spread3 {
-  static hipError_t run(const PointArgs &pa, const float2 *values, float2 *grid, const Geom &g,
-                        const KbPoly &poly, int nblk, hipStream_t s) {
+  static hipError_t run(const PointArgs &pa, const float *rec, const uint8_t *xo, float2 *grid,
+                        const Geom &g, const KbPoly &poly, int nblk, hipStream_t s) {
     size_t lds = spread_lds<W>(g);
     hipError_t e = hipFuncSetAttribute((const void *)spread3d_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(spread3d_kernel<W>, dim3(nblk), dim3(kSpreadThreads), lds, s, pa, values,
+    hipLaunchKernelGGL(spread3d_kernel<W>, dim3(nblk), dim3(kSpreadThreads), lds, s, pa, rec, xo,
                        grid, g, poly);
+    return hipGetLastError();
+  }
+};
+
+template <int W>
+struct Spread3Batched {
+  static hipError_t run(const SpreadProblem *probs, const uint32_t *prefix, int nprob,
+                        const KbPoly &poly, int nblk, hipStream_t s) {
+    const Geom g{};
+    size_t lds = spread_lds<W>(g);
+    hipError_t e = hipFuncSetAttribute((const void *)spread3d_batched_kernel<W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(spread3d_batched_kernel<W>, dim3(nblk), dim3(kSpreadThreads), lds, s,
+                       probs, prefix, nprob, poly);
     return hipGetLastError();
   }
 };
@@ -601,10 +728,35 @@ struct LdsSize {
 
 } // namespace
 
-hipError_t launch_spread3d(int w, const PointArgs &pa, const float2 *values, float2 *grid,
-                           const Geom &g, const KbPoly &poly, int nblk, hipStream_t s) {
-  return dispatch_w<Spread3>(w, pa, values, grid, g, poly, nblk, s);
+int spread_rec_floats(int w) { return (2 * w + 2 + 3) / 4 * 4; }
+
+hipError_t launch_spread_xrec(int w, const PointArgs &pa, const float2 *values, const Geom &g,
+                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s) {
+  return dispatch_w<SpreadXrec>(w, pa, values, g, poly, rec, xo, s);
 }
+
+hipError_t launch_spread3d(int w, const PointArgs &pa, const float *rec, const uint8_t *xo,
+                           float2 *grid, const Geom &g, const KbPoly &poly, int nblk,
+                           hipStream_t s) {
+  return dispatch_w<Spread3>(w, pa, rec, xo, grid, g, poly, nblk, s);
+}
+hipError_t launch_spread3d_batched(int w, const SpreadProblem *host_probs, int nprob,
+                                   SpreadProblem *dev_table, uint32_t *dev_prefix,
+                                   const KbPoly &poly, int nblk, hipStream_t s) {
+  for (int first = 0; first < nprob; first += kBatchChunk) {
+    BatchChunk chunk{};
+    chunk.first = first;
+    chunk.count = nprob - first < kBatchChunk ? nprob - first : kBatchChunk;
+    for (int i = 0; i < chunk.count; ++i) chunk.probs[i] = host_probs[first + i];
+    hipLaunchKernelGGL(batch_table_kernel, dim3(1), dim3(64), 0, s, chunk, dev_table);
+  }
+  hipLaunchKernelGGL(batch_prefix_kernel, dim3(1), dim3(64), 0, s, dev_table, nprob, dev_prefix);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return dispatch_w<Spread3Batched>(w, (const SpreadProblem *)dev_table, (const uint32_t *)dev_prefix,
+                                    nprob, poly, nblk, s);
+}
+
 hipError_t launch_interp3d(int w, const PointArgs &pa, const float2 *grid, float2 *out,
                            const Geom &g, const KbPoly &poly, int nblk, hipStream_t s) {
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);

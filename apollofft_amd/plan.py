@@ -280,8 +280,12 @@ class NufftPlan3D(_PlanBase):
         self.modes_shape = (int(nx), int(ny), int(nz))
 
 
+_BATCH_WS = {}
+
+
 def execute_type1_batched(plans, positions, values, outs=None):
-    """Batched Type-1 over independent 3D plans on torch's current stream."""
+    """Batched Type-1 over independent 3D plans on torch's current stream.
+    The workspace is cached per device and grown on demand."""
     import torch
     n = len(plans)
     if not (len(positions) == len(values) == n):
@@ -300,7 +304,10 @@ def execute_type1_batched(plans, positions, values, outs=None):
     Marr = (ctypes.c_int64 * n)(*Ms)
     b = ctypes.c_size_t()
     N.check(N.lib().anufft_batch_workspace_size(hs, n, Marr, ctypes.byref(b)), "batch_workspace_size")
-    ws = torch.empty(max(b.value, 256), dtype=torch.uint8, device=dev)
+    ws = _BATCH_WS.get(dev)
+    if ws is None or ws.numel() < b.value:
+        ws = torch.empty(max(b.value, 256), dtype=torch.uint8, device=dev)
+        _BATCH_WS[dev] = ws
     N.check(N.lib().anufft_execute_type1_batched(
         hs, n, Marr, N.ptr_array(coords), 3, N.ptr_array(vals),
         N.ptr_array([o.data_ptr() for o in outs]), ws.data_ptr(), ws.numel(),

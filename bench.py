@@ -74,6 +74,241 @@ def cpu_baseline(n, w, M_sample, seed=7):
     return 2.0 * M_sample / el / 1e6, el
 
 
+def clustered_inputs(rng, M, n):
+    """70% of the points from 8 Gaussian clusters (sigma 0.1, wrapped into
+    [-pi, pi)), 30% uniform; values and modes re/im ~ N(0,1)."""
+    mc = int(0.7 * M)
+    centres = rng.uniform(-np.pi, np.pi, size=(8, 3))
+    pts = np.empty((M, 3), dtype=np.float32)
+    which = rng.integers(0, 8, size=mc)
+    pts[:mc] = ((centres[which] + 0.1 * rng.standard_normal((mc, 3)) + np.pi) % TWO_PI - np.pi)
+    pts[mc:] = rng.uniform(-np.pi, np.pi, size=(M - mc, 3))
+    pts = pts[rng.permutation(M)]
+    c = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    f = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    return pts, c, f
+
+
+def time_pair(torch, plan, txyz, tc, tf, out1, out2, steps, warmup):
+    """Time Type-1 and Type-2 separately; returns (t1_s, t2_s) per call and
+    per-stage ms per call."""
+    for _ in range(warmup):
+        plan.type1(txyz, tc, out=out1)
+        plan.type2(txyz, tf, out=out2)
+    res = {}
+    plan.timing(True)
+    for name, fn in (("type1", lambda: plan.type1(txyz, tc, out=out1)),
+                     ("type2", lambda: plan.type2(txyz, tf, out=out2))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / steps
+    st = plan.timing_collect()
+    plan.timing(False)
+    return res, {k: v[0] / steps for k, v in st.items()}
+
+
+def check_errors(rng, xyz, c, f, n, d, out1, out2, nsel=4096):
+    import oracle
+    sel = rng.choice(int(np.prod(n)), size=nsel, replace=False).astype(np.int64)
+    o1 = out1.cpu().numpy().ravel()[sel]
+    r1 = oracle.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d, sel)
+    pj = rng.choice(xyz.shape[0], size=nsel, replace=False)
+    o2 = out2.cpu().numpy()[pj]
+    r2 = oracle.direct_type2_3d(f.astype(np.complex128), xyz[pj].astype(np.float64), d)
+    return {"type1": rel_l2(o1, r1), "type2": rel_l2(o2, r2)}
+
+
+def bench_sweep(args, torch, A, dev):
+    """Precision sweep: 100^3 -> 200^3 and 127^3 -> 254^3 grids, M = 16e6
+    clustered points, KB w in {4, 6, 8}; Type-1 and Type-2 Mpts/s and rel.
+    L2 error vs the FP64 direct sum on 4096 random modes / points."""
+    M = args.points or 16_000_000
+    widths = [args.w] if args.w else [4, 6, 8]
+    rows = []
+    for nn in (100, 127):
+        n = (nn, nn, nn)
+        d = (TWO_PI / nn,) * 3
+        rng = np.random.default_rng(nn)
+        xyz, c, f = clustered_inputs(rng, M, n)
+        txyz = torch.from_numpy(xyz).to(dev)
+        tc = torch.from_numpy(c).to(dev)
+        tf = torch.from_numpy(f).to(dev)
+        out1 = torch.empty(n, dtype=torch.complex64, device=dev)
+        out2 = torch.empty(M, dtype=torch.complex64, device=dev)
+        for w in widths:
+            plan = A.NufftPlan3D(*n, *d, 2, w, grid_policy="exact")
+            plan.workspace(M, dev)
+            per, st = time_pair(torch, plan, txyz, tc, tf, out1, out2, args.steps, args.warmup)
+            row = {"modes": nn, "grid": plan.meta["m"][0], "w": w, "points": M,
+                   "type1_mpts_per_s": round(M / per["type1"] / 1e6, 1),
+                   "type2_mpts_per_s": round(M / per["type2"] / 1e6, 1),
+                   "stage_ms": {k: round(v, 3) for k, v in st.items() if v}}
+            if not args.no_check:
+                row["rel_err_vs_fp64_direct"] = check_errors(rng, xyz, c, f, n, d, out1, out2)
+            rows.append(row)
+            print(json.dumps(row), file=sys.stderr, flush=True)
+            del plan
+            torch.cuda.empty_cache()
+    return rows
+
+
+def bench_grad(args, torch, A, dev):
+    """Gradient form: L = ||T2(f, x) - b||^2 with 64^3 modes, M = 1e6 uniform
+    points, w = 6; one step = forward Type-2 + backward (Type-1 for df and
+    the kernel-derivative interpolation for dx)."""
+    from apollofft_amd.autograd import nufft_type2
+    n = (64, 64, 64)
+    M = args.points or 1_000_000
+    w = args.w or 6
+    d = (TWO_PI / 64,) * 3
+    rng = np.random.default_rng(44)
+    xyz, _, f = make_inputs(rng, M, n)
+    b = (rng.standard_normal(M) + 1j * rng.standard_normal(M)).astype(np.complex64)
+    plan = A.NufftPlan3D(*n, *d, 2, w)
+    tx = torch.from_numpy(xyz).to(dev).requires_grad_(True)
+    tf = torch.from_numpy(f).to(dev).requires_grad_(True)
+    tb = torch.from_numpy(b).to(dev)
+
+    def step():
+        tx.grad = None
+        tf.grad = None
+        loss = (nufft_type2(tf, tx, plan) - tb).abs().pow(2).sum()
+        loss.backward()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.steps
+    row = {"modes": 64, "w": w, "points": M, "ms_per_step": round(el * 1e3, 3),
+           "mpts_per_s": round(M / el / 1e6, 1)}
+    if not args.no_check:
+        import oracle
+        gx = tx.grad.cpu().numpy().astype(np.float64)
+        gf = tf.grad.cpu().numpy()
+        pj = rng.choice(M, size=256, replace=False)
+        x64 = xyz.astype(np.float64)
+        f64 = f.astype(np.complex128)
+        r = oracle.direct_type2_3d(f64, x64[pj], d) - b[pj]
+        ref = oracle.direct_type2_3d_grad_x(f64, x64[pj], d, r)
+        row["rel_err_grad_x_vs_fp64_direct"] = rel_l2(gx[pj], ref)
+        sel = rng.choice(int(np.prod(n)), size=256, replace=False).astype(np.int64)
+        rfull = oracle.direct_type2_3d(f64, x64, d) - b if M <= 200_000 else None
+        if rfull is not None:
+            refg = 2.0 * oracle.direct_type1_3d(x64, rfull, n, d, sel)
+            row["rel_err_grad_modes_vs_fp64_direct"] = rel_l2(gf.ravel()[sel], refg)
+    return row
+
+
+def radial_points(rng, M):
+    """Points on ceil(M/512) lines through the origin with directions uniform
+    on the sphere and r uniform in [-pi, pi) (3D density ~ 1/r^2)."""
+    lines = -(-M // 512)
+    dirs = rng.standard_normal((lines, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    r = rng.uniform(-np.pi, np.pi, size=(lines, 512))
+    pts = (r[:, :, None] * dirs[:, None, :]).reshape(-1, 3)[:M]
+    return np.ascontiguousarray(pts, dtype=np.float32)
+
+
+def bench_batched(args, torch, A, dev):
+    """Batched Type-1: 64 independent problems, modes per problem drawn from
+    {32^3, 48^3, 64^3, 96^3} (grids 64^3, 96^3, 128^3, 192^3), M uniform in
+    [100k, 4M] on radial lines, w = 6.  One batched call (fused persistent
+    spreading, captured in a HIP graph) against 64 sequential plain calls."""
+    nprob = args.problems or 64
+    w = args.w or 6
+    rng = np.random.default_rng(55)
+    sizes = rng.choice([32, 48, 64, 96], size=nprob)
+    Ms = rng.integers(100_000, 4_000_001, size=nprob)
+    if args.points:
+        Ms = np.minimum(Ms, args.points)
+    plans, pos, vals, outs, host = [], [], [], [], []
+    plan_cache = {}
+    for nn, M in zip(sizes, Ms):
+        nn = int(nn)
+        if nn not in plan_cache:
+            plan_cache[nn] = A.NufftPlan3D(nn, nn, nn, *(TWO_PI / nn,) * 3, 2, w, grid_policy="exact")
+        plans.append(plan_cache[nn])
+        xyz = radial_points(rng, int(M))
+        c = (rng.standard_normal(int(M)) + 1j * rng.standard_normal(int(M))).astype(np.complex64)
+        host.append((xyz, c))
+        pos.append(torch.from_numpy(xyz).to(dev))
+        vals.append(torch.from_numpy(c).to(dev))
+        outs.append(torch.empty((nn, nn, nn), dtype=torch.complex64, device=dev))
+    total = int(Ms.sum())
+    # sequential plain launches (one workspace per distinct plan, as a user would)
+    seq_outs = [torch.empty_like(o) for o in outs]
+
+    def sequential():
+        for p, x, v, o in zip(plans, pos, vals, seq_outs):
+            p.type1(x, v, out=o)
+
+    for _ in range(args.warmup):
+        sequential()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sequential()
+    torch.cuda.synchronize()
+    t_seq = (time.perf_counter() - t0) / args.steps
+    # batched: one call, captured once into a HIP graph and replayed
+    A.execute_type1_batched(plans, pos, vals, outs)
+    torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            A.execute_type1_batched(plans, pos, vals, outs)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph):
+            A.execute_type1_batched(plans, pos, vals, outs)
+        torch.cuda.synchronize()
+
+    def batched():
+        if graph is not None:
+            graph.replay()
+        else:
+            A.execute_type1_batched(plans, pos, vals, outs)
+
+    for _ in range(args.warmup):
+        batched()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batched()
+    torch.cuda.synchronize()
+    t_bat = (time.perf_counter() - t0) / args.steps
+    row = {"problems": nprob, "points": total, "w": w, "graph": graph is not None,
+           "batched_ms": round(t_bat * 1e3, 3), "sequential_ms": round(t_seq * 1e3, 3),
+           "batched_mpts_per_s": round(total / t_bat / 1e6, 1),
+           "sequential_mpts_per_s": round(total / t_seq / 1e6, 1),
+           "speedup_vs_sequential": round(t_seq / t_bat, 4)}
+    diffs = [rel_l2(o.cpu().numpy(), so.cpu().numpy()) for o, so in zip(outs, seq_outs)]
+    row["max_rel_diff_batched_vs_sequential"] = max(diffs)
+    if not args.no_check:
+        import oracle
+        errs = []
+        for i in rng.choice(nprob, size=2, replace=False):
+            nn = int(sizes[i])
+            xyz, c = host[i]
+            sel = rng.choice(nn ** 3, size=512, replace=False).astype(np.int64)
+            ref = oracle.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128),
+                                         (nn,) * 3, (TWO_PI / nn,) * 3, sel)
+            errs.append(rel_l2(outs[i].cpu().numpy().ravel()[sel], ref))
+        row["rel_err_vs_fp64_direct_sampled"] = max(errs)
+    return row
+
+
 def bench_plain(args, torch, A, dev):
     n = (64, 64, 64)
     w = args.w or 7
@@ -137,7 +372,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="plain", choices=["plain"])
+    ap.add_argument("--config", default="plain", choices=["plain", "sweep", "grad", "batched"])
+    ap.add_argument("--problems", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--w", type=int, default=0)
     ap.add_argument("--points", type=int, default=0)
     ap.add_argument("--no-check", action="store_true")
@@ -167,6 +404,15 @@ def main():
     import torch
     import apollofft_amd as A
     dev = torch.device("cuda:0")
+    if args.config == "sweep":
+        print(json.dumps({"config": "sweep", "rows": bench_sweep(args, torch, A, dev)}), flush=True)
+        return
+    if args.config == "batched":
+        print(json.dumps({"config": "batched", **bench_batched(args, torch, A, dev)}), flush=True)
+        return
+    if args.config == "grad":
+        print(json.dumps({"config": "grad", **bench_grad(args, torch, A, dev)}), flush=True)
+        return
     r = bench_plain(args, torch, A, dev)
     M, K, el = r["M"], args.steps, r["elapsed"]
     st = r["stages"]

@@ -216,7 +216,8 @@ typedef enum anufft_stage {
   ANUFFT_STAGE_FFT = 3,    /* all FFT passes with pad/crop/deconvolution */
   ANUFFT_STAGE_INTERP = 4, /* Type-2 interpolation */
   ANUFFT_STAGE_GRAD = 5,   /* coordinate-gradient interpolation */
-  ANUFFT_STAGE_COUNT = 6
+  ANUFFT_STAGE_PREP = 6,   /* per-point spreading records (value, x-weights) */
+  ANUFFT_STAGE_COUNT = 7
 } anufft_stage;
 anufft_status anufft_timing_enable(const anufft_plan *plan, int32_t on);
 anufft_status anufft_timing_collect(const anufft_plan *plan, double *ms, int64_t *calls);
