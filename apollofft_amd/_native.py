@@ -44,6 +44,7 @@ EXPORTS = [
     "anufft_timing_enable", "anufft_timing_collect",
 ]
 
+SPREAD_AUTO, SPREAD_VALU, SPREAD_MFMA = 0, 1, 2
 STAGES = ["fill", "bin", "spread", "fft", "interp", "grad", "prep"]
 
 
@@ -101,6 +102,7 @@ class Opts(ctypes.Structure):
         ("device_id", ctypes.c_int32),
         ("bin_size", ctypes.c_int32 * 3),
         ("max_subproblem", ctypes.c_int32),
+        ("kernel_method", ctypes.c_int32),
     ]
 
 
@@ -122,6 +124,7 @@ class Meta(ctypes.Structure):
         ("interp_max_subproblem", ctypes.c_int32),
         ("poly_degree", ctypes.c_int32),
         ("poly_max_err", ctypes.c_double),
+        ("kernel_method", ctypes.c_int32),
     ]
 
 

@@ -12,7 +12,11 @@
 // modes of a line (zero-filling the rest in LDS) or write only the kept
 // modes, and the per-axis KB deconvolution factor (FP64 table) is applied
 // on the fly, so crop/pad/deconvolve never make a separate HBM pass.
-// Twiddles are FP64-built and rounded once to FP32.
+// Twiddles are FP64-built and rounded once to FP32.  Lengths with a prime
+// factor above 16 use Bluestein's algorithm: the DFT of length L is a
+// circular convolution of length Lf = 2^k >= 2L - 1 with the chirp
+// exp(-i pi n^2 / L), done as FFT, pointwise product with the precomputed
+// filter spectrum, FFT of the conjugate, all in the same LDS lines.
 #include "internal.h"
 
 namespace anufft {
@@ -154,16 +158,46 @@ __device__ __forceinline__ int kept_pos(int i, int nk, int L) {
   return (nk == L || i <= nk / 2) ? i : L - nk + i;
 }
 
+// All Stockham stages of one length-Lf transform of nl lines, ping-ponging
+// between *src and *dst (the result ends in *src).
+__device__ __forceinline__ void run_stages(const FftPass &p, int Lf, const float2 *tw, int nl,
+                                           float2 *&src, float2 *&dst) {
+  int Ns = 1;
+  for (int f = 0; f < p.nfac; ++f) {
+    __syncthreads();
+    const int R = p.fac[f];
+    switch (R) {
+    case 2: stage<2>(src, dst, Lf, Ns, tw, nl); break;
+    case 3: stage<3>(src, dst, Lf, Ns, tw, nl); break;
+    case 4: stage<4>(src, dst, Lf, Ns, tw, nl); break;
+    case 5: stage<5>(src, dst, Lf, Ns, tw, nl); break;
+    case 8: stage<8>(src, dst, Lf, Ns, tw, nl); break;
+    default: stage_generic(src, dst, Lf, Ns, R, tw, nl); break;
+    }
+    float2 *t = src;
+    src = dst;
+    dst = t;
+    Ns *= R;
+  }
+}
+
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   extern __shared__ float2 sh[];
   const int L = p.L, lb = p.lb;
+  const bool blue = p.blue_n > 0;
+  const int Lf = blue ? p.blue_n : L;  // transform length and line stride in LDS
   float2 *buf0 = sh;
-  float2 *buf1 = sh + (size_t)lb * L;
-  float2 *tw = sh + 2 * (size_t)lb * L;  // twiddle table of this length
-  for (int i = threadIdx.x; i < L; i += kFftThreads) tw[i] = p.tw[i];
+  float2 *buf1 = sh + (size_t)lb * Lf;
+  float2 *tw = sh + 2 * (size_t)lb * Lf;  // twiddle table of length Lf
+  for (int i = threadIdx.x; i < Lf; i += kFftThreads) tw[i] = p.tw[i];
   const int64_t l0 = (int64_t)blockIdx.x * lb;
   const int nl = (int)((p.nlines - l0) < lb ? (p.nlines - l0) : lb);
 
+  if (blue) {
+    // a_n = x_n chirp_n on [0, L), zero on [L, Lf): clear everything first
+    for (int idx = threadIdx.x; idx < nl * Lf; idx += kFftThreads) buf0[idx] = make_float2(0.f, 0.f);
+    __syncthreads();
+  }
   // ---- load (with deconvolution, conjugation for the inverse, zero pad) ----
   const bool in_contig = p.in_si == 1;
   for (int idx = threadIdx.x; idx < nl * p.nin; idx += kFftThreads) {
@@ -184,9 +218,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
       v.y *= f;
     }
     if (p.inverse) v.y = -v.y;
-    buf0[li * L + kept_pos(i, p.nin, L)] = v;
+    const int pos = kept_pos(i, p.nin, L);
+    if (blue) v = cmul(v, p.chirp[pos]);
+    buf0[li * Lf + pos] = v;
   }
-  if (p.nin < L) {
+  if (!blue && p.nin < L) {
     const int gap = L - p.nin, g0 = p.nin / 2 + 1;
     for (int idx = threadIdx.x; idx < nl * gap; idx += kFftThreads) {
       int li = idx / gap;
@@ -196,22 +232,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
 
   // ---- Stockham stages in LDS ----
   float2 *src = buf0, *dst = buf1;
-  int Ns = 1;
-  for (int f = 0; f < p.nfac; ++f) {
+  run_stages(p, Lf, tw, nl, src, dst);
+  if (blue) {
+    // C = FFT(a) FFT(b) / Lf; the inverse FFT of C is conj(FFT(conj(C)))
     __syncthreads();
-    const int R = p.fac[f];
-    switch (R) {
-    case 2: stage<2>(src, dst, L, Ns, tw, nl); break;
-    case 3: stage<3>(src, dst, L, Ns, tw, nl); break;
-    case 4: stage<4>(src, dst, L, Ns, tw, nl); break;
-    case 5: stage<5>(src, dst, L, Ns, tw, nl); break;
-    case 8: stage<8>(src, dst, L, Ns, tw, nl); break;
-    default: stage_generic(src, dst, L, Ns, R, tw, nl); break;
+    for (int idx = threadIdx.x; idx < nl * Lf; idx += kFftThreads) {
+      const float2 c = cmul(src[idx], p.bhat[idx % Lf]);
+      src[idx] = make_float2(c.x, -c.y);
     }
-    float2 *t = src;
-    src = dst;
-    dst = t;
-    Ns *= R;
+    run_stages(p, Lf, tw, nl, src, dst);
   }
   __syncthreads();
 
@@ -228,7 +257,9 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     }
     int64_t l = l0 + li;
     int64_t a = l / p.nb, b = l - a * p.nb;
-    float2 v = src[li * L + kept_pos(o, p.nout, L)];
+    const int pos = kept_pos(o, p.nout, L);
+    float2 v = src[li * Lf + pos];
+    if (blue) v = cmul(make_float2(v.x, -v.y), p.chirp[pos]);  // X_k = chirp_k conj(c_k)
     if (p.inverse) v.y = -v.y;
     float f = (float)(p.dec_out ? p.scale * p.dec_out[o] : p.scale);
     v.x *= f;
@@ -248,7 +279,13 @@ __global__ void fill_kernel(float2 *p, int64_t n) {
 hipError_t launch_fft_pass(const FftPass &p, hipStream_t s) {
   if (p.nlines <= 0) return hipSuccess;
   int64_t blocks = (p.nlines + p.lb - 1) / p.lb;
-  size_t lds = (2 * (size_t)p.lb + 1) * p.L * sizeof(float2);
+  const int Lf = p.blue_n > 0 ? p.blue_n : p.L;
+  size_t lds = (2 * (size_t)p.lb + 1) * Lf * sizeof(float2);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)fft_pass_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(fft_pass_kernel, dim3((unsigned)blocks), dim3(kFftThreads), lds, s, p);
   return hipGetLastError();
 }

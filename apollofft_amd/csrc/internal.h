@@ -28,6 +28,12 @@ constexpr int kSortChunk = kSortThreads * kSortItems;
 constexpr int kSpreadBinX = 4;      // spreading tile x-extent (a lane holds bin_x + 2w - 1 cells)
 constexpr int kSpreadCols = 32;     // spreading tile (y,z) columns per axis, bin + 2w <= 32
 
+// Spreading kernels: register columns on the vector ALUs, or FP16-split
+// MFMA blocks (values of anufft_opts.kernel_method).
+constexpr int kSpreadAuto = 0;
+constexpr int kSpreadValu = 1;
+constexpr int kSpreadMfma = 2;
+
 // Per-tap kernel polynomials, passed by value (kernarg segment -> scalar regs).
 // c[d][i] multiplies z^d for tap i; dc[d][i] is the same for d phi/dx.
 struct KbPoly {
@@ -72,11 +78,20 @@ struct FftPass {
   const double *dec_out; // factor per output index (nout) or null
   double scale;
   int inverse;           // 0: exp(-i), 1: exp(+i) (unnormalized)
-  const float2 *tw;      // exp(-2 pi i k / L), k in [0, L)
+  const float2 *tw;      // exp(-2 pi i k / Lf), k in [0, Lf)
   int nfac;
-  int fac[16];
+  int fac[16];           // radices of Lf
   int lb;                // lines per block
+  // Bluestein (lengths with a prime factor above kDirectPrimeMax): the
+  // length-L DFT is a circular convolution of length blue_n (a power of two,
+  // >= 2L - 1) done with two forward FFTs; Lf = blue_n, else Lf = L.
+  int blue_n;
+  const float2 *chirp;   // exp(-i pi n^2 / L), n in [0, L)
+  const float2 *bhat;    // FFT_blue_n of the conjugate chirp filter, times 1/blue_n
 };
+
+constexpr int kDirectPrimeMax = 16;   // larger prime radices go through Bluestein
+constexpr int kBluesteinMaxN = 4096;  // largest Bluestein FFT (one line in LDS)
 
 // Arguments shared by the 3D spread / interp kernels.
 struct PointArgs {
@@ -117,14 +132,15 @@ hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
 int spread_rec_floats(int w);
 hipError_t launch_spread_xrec(int w, const PointArgs &pa, const float2 *values, const Geom &g,
                               const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
-hipError_t launch_spread3d(int w, const PointArgs &pa, const float *rec, const uint8_t *xo,
-                           float2 *grid, const Geom &g, const KbPoly &poly, int num_blocks,
-                           hipStream_t s);
-hipError_t launch_spread3d_batched(int w, const SpreadProblem *host_probs, int nprob,
+hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
+                           const uint8_t *xo, float2 *grid, const Geom &g, const KbPoly &poly,
+                           int num_blocks, hipStream_t s);
+hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_probs, int nprob,
                                    SpreadProblem *dev_table, uint32_t *dev_prefix,
                                    const KbPoly &poly, int num_blocks, hipStream_t s);
-hipError_t launch_interp3d(int w, const PointArgs &pa, const float2 *grid, float2 *out,
-                           const Geom &g, const KbPoly &poly, int num_blocks, hipStream_t s);
+hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float2 *grid,
+                           float2 *out, const Geom &g, const KbPoly &poly, int num_blocks,
+                           hipStream_t s);
 hipError_t launch_grad3d(int w, const PointArgs &pa, const float2 *grid, const float2 *u,
                          float *gx, float *gy, float *gz, const Geom &g, const KbPoly &poly,
                          int num_blocks, hipStream_t s);
@@ -140,7 +156,9 @@ hipError_t launch_grad1d(int w, const float *x, int64_t stride, int64_t num_poin
                          const KbPoly &poly, hipStream_t s);
 
 // LDS bytes of the 3D spread/interp kernels for a geometry.
-size_t spread_lds_bytes(int w, const Geom &g);
+size_t spread_lds_bytes(int w, int method, const Geom &g);
+int spread_bin_x(int w, int method);        // spreading tile interior along x
+int spread_block_threads(int w, int method);
 size_t interp_lds_bytes(int w, const Geom &g, bool grad);
 
 // ---- device helpers ----------------------------------------------------------

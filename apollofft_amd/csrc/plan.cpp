@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -201,7 +202,10 @@ struct anufft_plan {
   std::vector<double> deconv[3];
   double *d_deconv[3] = {nullptr, nullptr, nullptr};
   float2 *d_tw[3] = {nullptr, nullptr, nullptr};
-  std::vector<int> fac[3];
+  std::vector<int> fac[3];          // radices of the in-LDS transform length per axis
+  int blue_n[3] = {0, 0, 0};        // Bluestein length per axis (0: direct mixed radix)
+  float2 *d_chirp[3] = {nullptr, nullptr, nullptr};
+  float2 *d_bhat[3] = {nullptr, nullptr, nullptr};
   KbPoly poly{};
   double poly_err = 0;
   // Binning geometries: spreading (register columns) and interpolation /
@@ -211,8 +215,9 @@ struct anufft_plan {
     int nbins = 1;
     int passes = 1;
     int max_sub = 1024;
-  } bs, bi;
+  } bs, bi, bg;
   int lds_max = 163840; // gfx950 LDS per workgroup
+  int kernel_method = kSpreadMfma;
   hipStream_t host_stream = nullptr;
   mutable StageTimer timer;
 
@@ -251,13 +256,14 @@ Layout layout(const anufft_plan *p, int64_t M) {
     l.idx1 = take(4 * Mp);
     int64_t cnt_n = (int64_t)kRadix * nch;
     l.cnt = take(4 * (size_t)cnt_n);
-    int64_t scan_n = std::max<int64_t>(cnt_n, std::max(p->bs.nbins, p->bi.nbins) + 1);
+    const int nb = std::max(p->bs.nbins, std::max(p->bi.nbins, p->bg.nbins));
+    int64_t scan_n = std::max<int64_t>(cnt_n, nb + 1);
     l.scan = take(4 * (size_t)((scan_n + 1023) / 1024 + 1));
-    const int nb = std::max(p->bs.nbins, p->bi.nbins);
     l.bin_start = take(4 * (size_t)(nb + 1));
     l.sub_ofs = take(4 * (size_t)(nb + 1));
-    int64_t nsub = std::max<int64_t>(p->bs.nbins + (M + p->bs.max_sub - 1) / p->bs.max_sub,
-                                     p->bi.nbins + (M + p->bi.max_sub - 1) / p->bi.max_sub) + 1;
+    int64_t nsub = 1;
+    for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
+      nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
     l.subs = take(16 * (size_t)nsub);
     l.rec = take(sizeof(float) * (size_t)spread_rec_floats(p->w) * Mp);
     l.xo = take(Mp);
@@ -268,6 +274,57 @@ Layout layout(const anufft_plan *p, int64_t M) {
   }
   l.total = off;
   return l;
+}
+
+hipError_t upload(float2 **dst, const std::vector<float2> &v, const char *what) {
+  hipError_t e = hipMalloc(dst, sizeof(float2) * v.size());
+  if (e == hipSuccess) e = hipMemcpy(*dst, v.data(), sizeof(float2) * v.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) (void)hip_fail(e, what);
+  return e;
+}
+
+// In-place FP64 radix-2 DFT, exp(-2 pi i nk / n), n a power of two.
+void fft_pow2_f64(std::vector<std::complex<double>> &x) {
+  const size_t n = x.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(x[i], x[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)len;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const std::complex<double> u = x[i + k], v = x[i + k + len / 2] * w;
+        x[i + k] = u + v;
+        x[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+// Bluestein tables of a length-L DFT done with length-n2 FFTs:
+// chirp_n = exp(-i pi n^2 / L) (n^2 reduced mod 2L in integers, so the
+// angle is exact before the FP64 sine/cosine), and the spectrum of the
+// filter b_m = conj(chirp_|m|) (m in (-L, L), circularly), scaled by 1/n2.
+void bluestein_tables(int L, int n2, std::vector<float2> &chirp, std::vector<float2> &bhat) {
+  std::vector<std::complex<double>> c(L), b(n2, 0.0);
+  for (int n = 0; n < L; ++n) {
+    const long long r = ((long long)n * n) % (2LL * L);
+    const double ang = -M_PI * (double)r / (double)L;
+    c[n] = std::complex<double>(std::cos(ang), std::sin(ang));
+  }
+  for (int m = 0; m < L; ++m) {
+    b[m] = std::conj(c[m]);
+    if (m > 0) b[n2 - m] = std::conj(c[m]);
+  }
+  fft_pow2_f64(b);
+  chirp.resize(L);
+  bhat.resize(n2);
+  for (int n = 0; n < L; ++n) chirp[n] = make_float2((float)c[n].real(), (float)c[n].imag());
+  for (int k = 0; k < n2; ++k)
+    bhat[k] = make_float2((float)(b[k].real() / n2), (float)(b[k].imag() / n2));
 }
 
 anufft_status build_tables(anufft_plan *p) {
@@ -299,7 +356,17 @@ anufft_status build_tables(anufft_plan *p) {
       err = std::max(err, std::fabs(horner_f32(p->poly.c, i, z) - kb_phi(x, wd, p->beta, p->i0b)));
     }
   p->poly_err = err;
-  for (int a = 0; a < p->dim; ++a) p->fac[a] = factorize((int)p->m[a]);
+  for (int a = 0; a < p->dim; ++a) {
+    const int L = (int)p->m[a];
+    p->fac[a] = factorize(L);
+    const int pmax = *std::max_element(p->fac[a].begin(), p->fac[a].end());
+    int n2 = 1;
+    while (n2 < 2 * L - 1) n2 <<= 1;
+    if (L > 1 && pmax > kDirectPrimeMax && n2 <= kBluesteinMaxN) {
+      p->blue_n[a] = n2;
+      p->fac[a] = factorize(n2);
+    }
+  }
   if (p->device < 0) return ANUFFT_OK;
 
   DeviceGuard guard(p->device);
@@ -311,16 +378,21 @@ anufft_status build_tables(anufft_plan *p) {
     e = hipMemcpy(p->d_deconv[a], p->deconv[a].data(), sizeof(double) * p->n[a],
                   hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy(deconv)");
-    int L = (int)p->m[a];
-    std::vector<float2> tw(L);
-    for (int k = 0; k < L; ++k) {
-      double ang = -2.0 * M_PI * (double)k / (double)L;
+    const int L = (int)p->m[a];
+    const int Lf = p->blue_n[a] ? p->blue_n[a] : L;
+    std::vector<float2> tw(Lf);
+    for (int k = 0; k < Lf; ++k) {
+      double ang = -2.0 * M_PI * (double)k / (double)Lf;
       tw[k] = make_float2((float)std::cos(ang), (float)std::sin(ang));
     }
-    e = hipMalloc(&p->d_tw[a], sizeof(float2) * L);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc(twiddles)");
-    e = hipMemcpy(p->d_tw[a], tw.data(), sizeof(float2) * L, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(twiddles)");
+    e = upload(&p->d_tw[a], tw, "twiddles");
+    if (e != hipSuccess) return ANUFFT_HIP_ERROR;
+    if (p->blue_n[a]) {
+      std::vector<float2> ch, bh;
+      bluestein_tables(L, Lf, ch, bh);
+      if (upload(&p->d_chirp[a], ch, "chirp") != hipSuccess) return ANUFFT_HIP_ERROR;
+      if (upload(&p->d_bhat[a], bh, "chirp filter") != hipSuccess) return ANUFFT_HIP_ERROR;
+    }
   }
   return ANUFFT_OK;
 }
@@ -332,6 +404,8 @@ void destroy_plan(anufft_plan *p) {
     for (int a = 0; a < 3; ++a) {
       if (p->d_deconv[a]) (void)hipFree(p->d_deconv[a]);
       if (p->d_tw[a]) (void)hipFree(p->d_tw[a]);
+      if (p->d_chirp[a]) (void)hipFree(p->d_chirp[a]);
+      if (p->d_bhat[a]) (void)hipFree(p->d_bhat[a]);
     }
     if (p->host_stream) (void)hipStreamDestroy(p->host_stream);
     for (auto &r : p->timer.recs) {
@@ -386,18 +460,20 @@ void choose_bins(anufft_plan *p) {
     const int b[3] = {(int)p->m[0], 1, 1};
     set_geom(p->bs, p, b);
     set_geom(p->bi, p, b);
-    p->bs.nbins = p->bi.nbins = 1;
+    set_geom(p->bg, p, b);
+    p->bs.nbins = p->bi.nbins = p->bg.nbins = 1;
     return;
   }
   const bool user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
   const int cyz = kSpreadCols - 2 * p->w;
-  const int bsp[3] = {kSpreadBinX, cyz, cyz};
+  const int bsp[3] = {spread_bin_x(p->w, p->kernel_method), cyz, cyz};
   set_geom(p->bs, p, bsp);
   p->bs.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 4096;
 
+  // LDS-tile geometry (gradient; interpolation on the vector-ALU path)
   const int cand[] = {32, 24, 16, 12, 10, 8, 6, 4, 3, 2, 1};
   if (user) {
-    set_geom(p->bi, p, p->opts.bin_size);
+    set_geom(p->bg, p, p->opts.bin_size);
   } else {
     double best = -1.0;
     int bb[3] = {1, 1, 1};
@@ -419,9 +495,15 @@ void choose_bins(anufft_plan *p) {
             bb[2] = bz;
           }
         }
-    set_geom(p->bi, p, bb);
+    set_geom(p->bg, p, bb);
   }
-  p->bi.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 1024;
+  p->bg.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 1024;
+  if (p->kernel_method == kSpreadMfma) {
+    set_geom(p->bi, p, bsp);  // MFMA interpolation uses the spreading tiles
+    p->bi.max_sub = p->bs.max_sub;
+  } else {
+    p->bi = p->bg;
+  }
 }
 
 anufft_status finish_create(anufft_plan *p, anufft_plan **out) {
@@ -435,9 +517,9 @@ anufft_status finish_create(anufft_plan *p, anufft_plan **out) {
   p->i0b = kb_i0(p->beta);
   choose_bins(p);
   if (p->dim == 3) {
-    size_t need = std::max(spread_lds_bytes(p->w, p->bs.g),
-                           std::max(interp_lds_bytes(p->w, p->bi.g, false),
-                                    interp_lds_bytes(p->w, p->bi.g, true)));
+    size_t need = std::max(spread_lds_bytes(p->w, p->kernel_method, p->bs.g),
+                           std::max(interp_lds_bytes(p->w, p->bg.g, false),
+                                    interp_lds_bytes(p->w, p->bg.g, true)));
     if (need > (size_t)p->lds_max) {
       destroy_plan(p);
       return fail(ANUFFT_INVALID_ARG, "bin_size needs %zu bytes of LDS, device has %d", need,
@@ -504,9 +586,13 @@ FftPass make_pass(const anufft_plan *p, int axis, const float2 *in, float2 *out,
   f.tw = p->d_tw[axis];
   f.nfac = (int)p->fac[axis].size();
   for (int i = 0; i < f.nfac && i < 16; ++i) f.fac[i] = p->fac[axis][i];
-  // lines per block: two LDS buffers of lb*L complex within 64 KiB, and
+  f.blue_n = p->blue_n[axis];
+  f.chirp = p->d_chirp[axis];
+  f.bhat = p->d_bhat[axis];
+  // lines per block: two LDS buffers of lb*Lf complex within 64 KiB, and
   // enough blocks to cover the chip when there are few lines.
-  int lb = std::max(1, 4096 / f.L);
+  const int Lf = f.blue_n ? f.blue_n : f.L;
+  int lb = std::max(1, 4096 / Lf);
   while (lb > 8 && (f.nlines + lb - 1) / lb < 2 * (int64_t)p->cus) lb >>= 1;
   f.lb = (int)std::max<int64_t>(1, std::min<int64_t>(lb, f.nlines));
   return f;
@@ -540,7 +626,7 @@ struct Exec {
     });
     if (e != hipSuccess) return e;
     return timed(ANUFFT_STAGE_SPREAD, [&] {
-      return launch_spread3d(p->w, pa, rec(), xo(), grid, p->bs.g, p->poly,
+      return launch_spread3d(p->w, p->kernel_method, pa, rec(), xo(), grid, p->bs.g, p->poly,
                              blocks(p->bs, pa.num_points, 2), s);
     });
   }
@@ -728,6 +814,12 @@ anufft_status anufft_plan3d_create(int64_t nx, int64_t ny, int64_t nz, double dx
     delete p;
     return fail(ANUFFT_U32_OVERFLOW, "oversampled grid size exceeds u32");
   }
+  const int sm = p->opts.kernel_method;
+  if (sm < kSpreadAuto || sm > kSpreadMfma) {
+    delete p;
+    return fail(ANUFFT_INVALID_ARG, "kernel_method %d is not 0 (auto), 1 (valu) or 2 (mfma)", sm);
+  }
+  p->kernel_method = sm == kSpreadAuto ? kSpreadMfma : sm;
   return finish_create(p, plan);
 }
 
@@ -757,6 +849,7 @@ anufft_status anufft_plan_get_meta(const anufft_plan *p, anufft_meta *meta) {
   meta->interp_max_subproblem = p->bi.max_sub;
   meta->poly_degree = kPolyCoefs - 1;
   meta->poly_max_err = p->poly_err;
+  meta->kernel_method = p->kernel_method;
   return ANUFFT_OK;
 }
 
@@ -822,8 +915,9 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp3d(p->w, pa, ex.grid(), (float2 *)out, p->bi.g, p->poly,
-                             ex.blocks(p->bi, M, 4), ex.s);
+      return launch_interp3d(p->w, p->kernel_method, pa, ex.grid(), (float2 *)out, p->bi.g,
+                             p->poly, ex.blocks(p->bi, M, p->kernel_method == kSpreadMfma ? 2 : 4),
+                             ex.s);
     });
   } else {
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes, false); });
@@ -852,14 +946,14 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
   hipError_t e;
   // the derivative always uses the unnormalized (direct-sum) scaling
   if (p->dim == 3) {
-    PointArgs pa = ex.point_args(p->bi, M, coords, stride);
-    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bi.g, 3, ex.sort_buffers(p->bi), ex.s); });
+    PointArgs pa = ex.point_args(p->bg, M, coords, stride);
+    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bg.g, 3, ex.sort_buffers(p->bg), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
       return launch_grad3d(p->w, pa, ex.grid(), (const float2 *)u, grad[0], grad[1], grad[2],
-                           p->bi.g, p->poly, ex.blocks(p->bi, M, 4), ex.s);
+                           p->bg.g, p->poly, ex.blocks(p->bg, M, 4), ex.s);
     });
   } else {
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
@@ -999,7 +1093,9 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
     if (!out[i] || (num_points[i] > 0 && (!values[i] || !coords[3 * i] || !coords[3 * i + 1] || !coords[3 * i + 2])))
       return fail(ANUFFT_INVALID_ARG, "NULL pointer in problem %d", i);
     if (num_points[i] > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points[%d] exceeds u32", i);
-    if (plans[i]->w != plans[0]->w || plans[i]->sigma != plans[0]->sigma) fused = false;
+    if (plans[i]->w != plans[0]->w || plans[i]->sigma != plans[0]->sigma ||
+        plans[i]->kernel_method != plans[0]->kernel_method)
+      fused = false;
   }
   const anufft_plan *p0 = plans[0];
   DeviceGuard guard(p0->device);
@@ -1065,7 +1161,8 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   if (e != hipSuccess) return hip_fail(e, "spread records");
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(sub_upper, 2 * (int64_t)p0->cus));
   e = e0.timed(ANUFFT_STAGE_SPREAD, [&] {
-    return launch_spread3d_batched(p0->w, probs.data(), nprob, table, prefix, p0->poly, nblk, s);
+    return launch_spread3d_batched(p0->w, p0->kernel_method, probs.data(), nprob, table, prefix,
+                                   p0->poly, nblk, s);
   });
   if (e != hipSuccess) return hip_fail(e, "spread");
   e = e0.timed(ANUFFT_STAGE_FFT, [&] {

@@ -299,6 +299,288 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa, 
     spread_subproblem<W>(pa, rec, xo, grid, g, pa.subs[sidx], sh);
 }
 
+// ---- MFMA spreading ---------------------------------------------------------
+//
+// The tile's (y,z) plane is a 32 x 32 MFMA output block per x cell: for x
+// cell i, G_i[y][z] += sum_k A_i[y][k] B[k][z] with A_i[y][k] = wx_k(i) wy_k(y)
+// and B[k][z] = c_k wz_k(z) (complex c: one block for re, one for im), k over
+// the subproblem's points 16 at a time (v_mfma_f32_32x32x16_f16).  Each FP32
+// operand is split into FP16 hi + lo (hi = fp16(a), lo = fp16(a - hi)) and a
+// product is hi*hi + hi*lo + lo*hi, accumulated in FP32 by the MFMA: about
+// 22 significant bits per product.  The strengths of a subproblem are scaled
+// by a power of two so that max |c| is in [1/2, 1), which keeps every
+// operand inside FP16's range; the flush undoes the scale exactly.  A wave
+// owns XW = 4 consecutive x cells (4 x 2 accumulator blocks); the B
+// operands of a point group are shared by the wave's cells, A is rebuilt per
+// cell from the staged y and x weights.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int W>
+struct MSpread {
+  static constexpr int NW = W >= 8 ? 8 : 4;    // waves per workgroup
+  static constexpr int XW = 4;                 // x cells per wave
+  static constexpr int PX = NW * XW;           // x cells of a tile
+  static constexpr int BINX = PX - 2 * W + 1;  // tile interior along x
+  static constexpr int NT = 64 * NW;
+};
+constexpr int kMNP = 128;       // points staged per round
+constexpr int kMWY = kMNP + 4;  // row stride (floats) of the y / x weight tables
+constexpr int kMBT = kMNP + 8;  // row stride (halves) of the B operand tables
+// LDS layout of the MFMA spreading kernels (bytes), for a tile of PX x cells.
+template <int PX>
+struct MLds {
+  static constexpr int WY = 0;                                 // float [32][kMWY]
+  static constexpr int WX = WY + 32 * kMWY * 4;                // float [PX][kMWY]
+  static constexpr int BT = WX + PX * kMWY * 4;                // half [4][32][kMBT]
+  static constexpr int C = BT + 4 * 32 * kMBT * 2;             // float2 [kMNP]
+  static constexpr int WZ = C + kMNP * 8;                      // float [kMNP][32]
+  static constexpr int PC = WZ + kMNP * 32 * 4;                // float [kPolyCoefs][kMaxTaps]
+  static constexpr int RED = PC + kPolyCoefs * kMaxTaps * 4;   // float [8]
+  static constexpr int SIZE = RED + 8 * 4;
+};
+// two 4-wave workgroups per CU for w <= 7
+static_assert(2 * MLds<16>::SIZE <= 160 * 1024, "MFMA spreading LDS");
+
+__device__ __forceinline__ void split8(const float *a, half8 &hi, half8 &lo) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const _Float16 h = (_Float16)a[j];
+    hi[j] = h;
+    lo[j] = (_Float16)(a[j] - (float)h);
+  }
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Tap weights of one axis written into a zero-filled LDS row or column:
+// dst[(l + i) * step] = phi_i for i in [0, 2w), from the coefficients pc.
+template <int W>
+__device__ __forceinline__ void put_taps(float *dst, int step, int l, float z, const float *pc) {
+#pragma unroll 1
+  for (int i = 0; i < 2 * W; ++i) {
+    float v = pc[(kPolyCoefs - 1) * kMaxTaps + i];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, pc[d * kMaxTaps + i]);
+    if (i == 0 && z <= -1.0f) v = 0.0f;
+    if (i == 2 * W - 1 && z >= 1.0f) v = 0.0f;
+    dst[(l + i) * step] = v;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, const float *rec,
+                                                       const uint8_t *xo, float2 *grid,
+                                                       const Geom &g, int4 sub, char *sh) {
+  using MS = MSpread<W>;
+  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, T = 2 * W;
+  constexpr int RW = XRec<W>::RW;
+  using L = MLds<PX>;
+  float *WY = (float *)(sh + L::WY);
+  float *WX = (float *)(sh + L::WX);
+  _Float16 *BT = (_Float16 *)(sh + L::BT);
+  float2 *C = (float2 *)(sh + L::C);
+  float *WZ = (float *)(sh + L::WZ);
+  const float *pc = (const float *)(sh + L::PC);
+  float *red = (float *)(sh + L::RED);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+
+  const int bin = sub.x;
+  const int bz = bin % g.nbin[2];
+  const int by = (bin / g.nbin[2]) % g.nbin[1];
+  const int bx = bin / (g.nbin[2] * g.nbin[1]);
+  const int bx0 = bx * g.bin[0];
+  const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
+
+  // power-of-two scale of the strengths: max |c| * 2^-e in [1/2, 1)
+  float mx = 0.f;
+  for (int p = threadIdx.x; p < sub.z; p += NT) {
+    const float *r = rec + (int64_t)(sub.y + p) * RW;
+    mx = fmaxf(mx, fmaxf(fabsf(r[0]), fabsf(r[1])));
+  }
+  mx = wave_max(mx);
+  __syncthreads();  // previous subproblem's readers of red[] are done
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  mx = red[0];
+#pragma unroll
+  for (int i = 1; i < MS::NW; ++i) mx = fmaxf(mx, red[i]);
+  int e = 0;
+  if (mx > 0.f) (void)frexpf(mx, &e);
+  e = max(-120, min(120, e));  // both scales stay finite and normal
+  const float s_in = ldexpf(1.0f, -e), s_out = ldexpf(1.0f, e);
+
+  f32x16 acc[XW][2];
+#pragma unroll
+  for (int q = 0; q < XW; ++q) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      acc[q][0][r] = 0.f;
+      acc[q][1][r] = 0.f;
+    }
+  }
+
+  for (int c0 = 0; c0 < sub.z; c0 += kMNP) {
+    const int npts = min(kMNP, sub.z - c0);
+    const int ng = (npts + 15) >> 4;
+    const int np16 = ng << 4;
+    const int start = sub.y + c0;
+    __syncthreads();  // the previous round's operand reads are done
+    // pass 1: x weights and strengths (from the records), y and z weights
+    for (int t = threadIdx.x; t < 3 * np16; t += NT) {
+      const int ax = t / np16;
+      const int p = t - ax * np16;
+      if (ax == 0) {
+#pragma unroll
+        for (int i = 0; i < PX; ++i) WX[i * kMWY + p] = 0.f;
+        if (p < npts) {
+          const float *r = rec + (int64_t)(start + p) * RW;
+          C[p] = make_float2(r[0] * s_in, r[1] * s_in);
+          const int o = xo[start + p];
+#pragma unroll
+          for (int i = 0; i < T; ++i) WX[(o + i) * kMWY + p] = r[2 + i];
+        } else {
+          C[p] = make_float2(0.f, 0.f);
+        }
+      } else if (ax == 1) {
+#pragma unroll
+        for (int y = 0; y < 32; ++y) WY[y * kMWY + p] = 0.f;
+        if (p < npts) {
+          const uint32_t j = pa.perm[start + p];
+          int cell;
+          const double tt = fold_coord(pa.y[(int64_t)j * pa.stride], g.scale[1], g.m[1], cell);
+          const float z = (float)(1.0 - 2.0 * (tt - (double)cell));
+          put_taps<W>(WY + p, kMWY, cell - W + 1 - oy, z, pc);
+        }
+      } else {
+        float *row = WZ + p * 32;
+#pragma unroll
+        for (int zz = 0; zz < 32; zz += 4) *(float4 *)(row + zz) = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (p < npts) {
+          const uint32_t j = pa.perm[start + p];
+          int cell;
+          const double tt = fold_coord(pa.z[(int64_t)j * pa.stride], g.scale[2], g.m[2], cell);
+          const float z = (float)(1.0 - 2.0 * (tt - (double)cell));
+          put_taps<W>(row, 1, cell - W + 1 - oz, z, pc);
+        }
+      }
+    }
+    __syncthreads();
+    // pass 2: B operands c_k wz_k(z), split, 8 points per task
+    for (int t = threadIdx.x; t < 32 * (np16 >> 3); t += NT) {
+      const int z = t & 31, pb = (t >> 5) << 3;
+      float br[8], bi[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float w = WZ[(pb + j) * 32 + z];
+        const float2 c = C[pb + j];
+        br[j] = c.x * w;
+        bi[j] = c.y * w;
+      }
+      half8 rh, rl, ih, il;
+      split8(br, rh, rl);
+      split8(bi, ih, il);
+      *(half8 *)(BT + (0 * 32 + z) * kMBT + pb) = rh;
+      *(half8 *)(BT + (1 * 32 + z) * kMBT + pb) = rl;
+      *(half8 *)(BT + (2 * 32 + z) * kMBT + pb) = ih;
+      *(half8 *)(BT + (3 * 32 + z) * kMBT + pb) = il;
+    }
+    __syncthreads();
+    // MFMA: 16 points per step; lane (l32, h) holds A row y = l32 and
+    // B column z = l32 for the points kb .. kb + 7
+    for (int gi = 0; gi < ng; ++gi) {
+      const int kb = (gi << 4) + (h << 3);
+      const half8 brh = *(const half8 *)(BT + (0 * 32 + l32) * kMBT + kb);
+      const half8 brl = *(const half8 *)(BT + (1 * 32 + l32) * kMBT + kb);
+      const half8 bih = *(const half8 *)(BT + (2 * 32 + l32) * kMBT + kb);
+      const half8 bil = *(const half8 *)(BT + (3 * 32 + l32) * kMBT + kb);
+      float wy[8];
+      *(float4 *)wy = *(const float4 *)(WY + l32 * kMWY + kb);
+      *(float4 *)(wy + 4) = *(const float4 *)(WY + l32 * kMWY + kb + 4);
+#pragma unroll
+      for (int q = 0; q < XW; ++q) {
+        const int ix = wave * XW + q;
+        float a[8];
+        *(float4 *)a = *(const float4 *)(WX + ix * kMWY + kb);
+        *(float4 *)(a + 4) = *(const float4 *)(WX + ix * kMWY + kb + 4);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[j] *= wy[j];
+        half8 ah, al;
+        split8(a, ah, al);
+        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, brh, acc[q][0], 0, 0, 0);
+        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, brl, acc[q][0], 0, 0, 0);
+        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, brh, acc[q][0], 0, 0, 0);
+        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bih, acc[q][1], 0, 0, 0);
+        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bil, acc[q][1], 0, 0, 0);
+        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bih, acc[q][1], 0, 0, 0);
+      }
+    }
+  }
+
+  // flush: accumulator r of lane (l32, h) is tile cell (y, z) with
+  // y = 8 (r / 4) + 4 h + r % 4, z = l32; x cell ix <-> x = bx0 - W + 1 + ix
+  if (l32 < g.pad[2]) {
+    const int gz = wrap(oz + l32, g.m[2]);
+#pragma unroll
+    for (int q = 0; q < XW; ++q) {
+      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int y = ((r >> 2) << 3) + (h << 2) + (r & 3);
+        const float vr = acc[q][0][r] * s_out, vi = acc[q][1][r] * s_out;
+        if (y < g.pad[1] && (vr != 0.f || vi != 0.f)) {
+          const int gy = wrap(oy + y, g.m[1]);
+          float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
+          unsafeAtomicAdd(&dst->x, vr);
+          unsafeAtomicAdd(&dst->y, vi);
+        }
+      }
+    }
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void mspread_load_poly(const KbPoly &poly, char *sh) {
+  float *pc = (float *)(sh + MLds<MSpread<W>::PX>::PC);
+  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += blockDim.x)
+    pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
+}
+
+template <int W>
+__global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
+    PointArgs pa, const float *rec, const uint8_t *xo, float2 *grid, Geom g, KbPoly poly) {
+  extern __shared__ __align__(16) char shm[];
+  mspread_load_poly<W>(poly, shm);  // ordered before use by the staging barriers
+  const uint32_t nsub = *pa.nsub;
+  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x)
+    spread_subproblem_mfma<W>(pa, rec, xo, grid, g, pa.subs[sidx], shm);
+}
+
+template <int W>
+__global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kernel(
+    const SpreadProblem *probs, const uint32_t *prefix, int nprob, KbPoly poly) {
+  extern __shared__ __align__(16) char shm[];
+  mspread_load_poly<W>(poly, shm);
+  const uint32_t total = prefix[nprob];
+  for (uint32_t gidx = blockIdx.x; gidx < total; gidx += gridDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= gidx) lo = mid;
+      else hi = mid - 1;
+    }
+    const SpreadProblem &pr = probs[lo];
+    spread_subproblem_mfma<W>(pr.pa, pr.rec, pr.xo, pr.grid, pr.g, pr.pa.subs[gidx - prefix[lo]],
+                              shm);
+  }
+}
+
 // Batched spreading: one persistent launch over the subproblems of all
 // problems.  prefix[p] is the first global subproblem index of problem p
 // (prefix[nprob] = total); the problem of a global index is found by a
@@ -352,6 +634,212 @@ __device__ __forceinline__ void load_tile(const float2 *grid, const Geom &g, int
     const int gy = wrap(oy + py, g.m[1]);
     const int gz = wrap(oz + pz, g.m[2]);
     tile[i] = grid[((int64_t)gx * g.m[1] + gy) * g.m[2] + gz];
+  }
+}
+
+// ---- MFMA interpolation -----------------------------------------------------
+//
+// Same tile geometry as MFMA spreading.  For x cell i the wave computes
+// D_i[y][k] = sum_z G_i[y][z] wz_k(z) (A = the tile plane, 32 y x 32 z, held
+// in registers for the whole subproblem; B = the z weights of 32 points),
+// then out_k += wx_k(i) sum_y wy_k(y) D_i[y][k] on the vector ALUs.  The
+// tile is scaled by a power of two so that its largest component is in
+// [1/2, 1) and split into FP16 hi + lo like the spreading operands.
+constexpr int kINP = 128;       // points staged per round (4 MFMA groups of 32)
+constexpr int kIRow = 36;       // row stride (floats) of the staged weight rows
+
+template <int PX>
+struct ILds {
+  static constexpr int WZ = 0;                                 // float [kINP][kIRow]
+  static constexpr int WY = WZ + kINP * kIRow * 4;             // float [kINP][kIRow]
+  static constexpr int WX = WY + kINP * kIRow * 4;             // float [kINP][kIRow]
+  static constexpr int RED = WX + kINP * kIRow * 4;            // float2 [8][kINP]
+  static constexpr int JIDX = RED + 8 * kINP * 8;              // int [kINP]
+  static constexpr int PC = JIDX + kINP * 4;                   // float [kPolyCoefs][kMaxTaps]
+  static constexpr int TMX = PC + kPolyCoefs * kMaxTaps * 4;   // float [8]
+  static constexpr int SIZE = TMX + 8 * 4;
+};
+static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
+
+// Weights of one axis into a zero-filled staged row: row[l + i] = phi_i.
+template <int W>
+__device__ __forceinline__ void stage_axis(float *row, int l, double t, int cell, const float *pc) {
+#pragma unroll
+  for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+  const float z = (float)(1.0 - 2.0 * (t - (double)cell));
+  put_taps<W>(row, 1, l, z, pc);
+}
+
+template <int W>
+__global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointArgs pa,
+                                                                          const float2 *grid,
+                                                                          float2 *out, Geom g,
+                                                                          KbPoly poly) {
+  using MS = MSpread<W>;
+  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW;
+  using L = ILds<PX>;
+  extern __shared__ __align__(16) char shm[];
+  float *WZ = (float *)(shm + L::WZ);
+  float *WY = (float *)(shm + L::WY);
+  float *WX = (float *)(shm + L::WX);
+  float2 *RED = (float2 *)(shm + L::RED);
+  int *JIDX = (int *)(shm + L::JIDX);
+  float *pc = (float *)(shm + L::PC);
+  float *tmx = (float *)(shm + L::TMX);
+  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += NT) pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  const uint32_t nsub = *pa.nsub;
+  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x) {
+    const int4 sub = pa.subs[sidx];
+    const int bin = sub.x;
+    const int bz = bin % g.nbin[2];
+    const int by = (bin / g.nbin[2]) % g.nbin[1];
+    const int bx = bin / (g.nbin[2] * g.nbin[1]);
+    const int bx0 = bx * g.bin[0];
+    const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
+
+    // A operands: lane (l32, h) holds G[x][y = l32][z] for z = 8h + j (step
+    // 0) and 16 + 8h + j (step 1) of its XW x cells; columns outside the
+    // padded tile are zero.
+    // The tile is read twice (the second time from L2): once for its
+    // largest component, once to scale and split, so the raw values need no
+    // registers of their own.
+    float mx = 0.f;
+    const bool yok = l32 < g.pad[1];
+    const int gy = wrap(oy + l32, g.m[1]);
+#pragma unroll
+    for (int q = 0; q < XW; ++q) {
+      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+      const float2 *rowp = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int z = (j < 8 ? 0 : 8) + 8 * h + j;
+        if (yok && z < g.pad[2]) {
+          const float2 v = rowp[wrap(oz + z, g.m[2])];
+          mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+      }
+    }
+    mx = wave_max(mx);
+    __syncthreads();  // previous subproblem's readers of tmx / RED are done
+    if (lane == 0) tmx[wave] = mx;
+    __syncthreads();
+    mx = tmx[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) mx = fmaxf(mx, tmx[i]);
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);
+    e = max(-120, min(120, e));
+    const float s_in = ldexpf(1.0f, -e), s_out = ldexpf(1.0f, e);
+    half8 arh[XW][2], arl[XW][2], aih[XW][2], ail[XW][2];
+#pragma unroll
+    for (int q = 0; q < XW; ++q) {
+      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+      const float2 *rowp = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2];
+#pragma unroll
+      for (int st = 0; st < 2; ++st) {
+        float tr[8], ti[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int z = 16 * st + 8 * h + j;
+          float2 v = make_float2(0.f, 0.f);
+          if (yok && z < g.pad[2]) v = rowp[wrap(oz + z, g.m[2])];
+          tr[j] = v.x * s_in;
+          ti[j] = v.y * s_in;
+        }
+        split8(tr, arh[q][st], arl[q][st]);
+        split8(ti, aih[q][st], ail[q][st]);
+      }
+    }
+
+    for (int c0 = 0; c0 < sub.z; c0 += kINP) {
+      const int npts = min(kINP, sub.z - c0);
+      const int ng = (npts + 31) >> 5;
+      const int np32 = ng << 5;
+      const int start = sub.y + c0;
+      __syncthreads();  // the previous round's staged rows and RED are consumed
+      for (int t = threadIdx.x; t < 3 * np32; t += NT) {
+        const int ax = t / np32;
+        const int p = t - ax * np32;
+        float *row = (ax == 0 ? WX : (ax == 1 ? WY : WZ)) + p * kIRow;
+        if (p < npts) {
+          const uint32_t j = pa.perm[start + p];
+          const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
+          int cell;
+          const double tt = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
+          // x: register cell of the first tap; y, z: tile column of the first tap
+          const int l = ax == 0 ? cell - bx0 : cell - W + 1 - (ax == 1 ? oy : oz);
+          stage_axis<W>(row, l, tt, cell, pc);
+          if (ax == 0) JIDX[p] = (int)j;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+      __syncthreads();
+      for (int gi = 0; gi < ng; ++gi) {
+        const int k = (gi << 5) + l32;  // this lane's point
+        half8 bh[2], bl[2];
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+          float wz[8];
+          *(float4 *)wz = *(const float4 *)(WZ + k * kIRow + 16 * st + 8 * h);
+          *(float4 *)(wz + 4) = *(const float4 *)(WZ + k * kIRow + 16 * st + 8 * h + 4);
+          split8(wz, bh[st], bl[st]);
+        }
+        // wy at the D rows of this lane: y = 8 (r / 4) + 4 h + r % 4
+        float wy[16];
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4)
+          *(float4 *)(wy + 4 * r4) = *(const float4 *)(WY + k * kIRow + 8 * r4 + 4 * h);
+        float wx[XW];
+#pragma unroll
+        for (int q = 0; q < XW; ++q) wx[q] = WX[k * kIRow + wave * XW + q];
+        float accr = 0.f, acci = 0.f;
+#pragma unroll
+        for (int q = 0; q < XW; ++q) {
+          f32x16 dr, di;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            dr[r] = 0.f;
+            di[r] = 0.f;
+          }
+#pragma unroll
+          for (int st = 0; st < 2; ++st) {
+            dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bh[st], dr, 0, 0, 0);
+            dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bl[st], dr, 0, 0, 0);
+            dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl[q][st], bh[st], dr, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bh[st], di, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bl[st], di, 0, 0, 0);
+            di = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail[q][st], bh[st], di, 0, 0, 0);
+          }
+          float pr = 0.f, pi = 0.f;
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            pr = fmaf(wy[r], dr[r], pr);
+            pi = fmaf(wy[r], di[r], pi);
+          }
+          accr = fmaf(wx[q], pr, accr);
+          acci = fmaf(wx[q], pi, acci);
+        }
+        accr += __shfl_xor(accr, 32);
+        acci += __shfl_xor(acci, 32);
+        if (h == 0) RED[wave * kINP + k] = make_float2(accr, acci);
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < npts; p += NT) {
+        float2 v = RED[p];
+#pragma unroll
+        for (int w2 = 1; w2 < NW; ++w2) {
+          const float2 u = RED[w2 * kINP + p];
+          v.x += u.x;
+          v.y += u.y;
+        }
+        out[JIDX[p]] = make_float2(v.x * s_out, v.y * s_out);
+      }
+    }
   }
 }
 
@@ -637,6 +1125,50 @@ struct Spread3 {
 };
 
 template <int W>
+struct Spread3Mfma {
+  static hipError_t run(const PointArgs &pa, const float *rec, const uint8_t *xo, float2 *grid,
+                        const Geom &g, const KbPoly &poly, int nblk, hipStream_t s) {
+    constexpr int lds = MLds<MSpread<W>::PX>::SIZE;
+    hipError_t e = hipFuncSetAttribute((const void *)spread3d_mfma_kernel<W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(spread3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
+                       rec, xo, grid, g, poly);
+    return hipGetLastError();
+  }
+};
+
+template <int W>
+struct Spread3MfmaBatched {
+  static hipError_t run(const SpreadProblem *probs, const uint32_t *prefix, int nprob,
+                        const KbPoly &poly, int nblk, hipStream_t s) {
+    constexpr int lds = MLds<MSpread<W>::PX>::SIZE;
+    hipError_t e = hipFuncSetAttribute((const void *)spread3d_mfma_batched_kernel<W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(spread3d_mfma_batched_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds,
+                       s, probs, prefix, nprob, poly);
+    return hipGetLastError();
+  }
+};
+
+template <int W>
+struct SpreadBinX {
+  static hipError_t run(int method, int *out) {
+    *out = method == kSpreadMfma ? MSpread<W>::BINX : kSpreadBinX;
+    return hipSuccess;
+  }
+};
+
+template <int W>
+struct MSpreadLds {
+  static hipError_t run(size_t *out) {
+    *out = std::max<size_t>(MLds<MSpread<W>::PX>::SIZE, ILds<MSpread<W>::PX>::SIZE);
+    return hipSuccess;
+  }
+};
+
+template <int W>
 struct Spread3Batched {
   static hipError_t run(const SpreadProblem *probs, const uint32_t *prefix, int nprob,
                         const KbPoly &poly, int nblk, hipStream_t s) {
@@ -647,6 +1179,20 @@ struct Spread3Batched {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(spread3d_batched_kernel<W>, dim3(nblk), dim3(kSpreadThreads), lds, s,
                        probs, prefix, nprob, poly);
+    return hipGetLastError();
+  }
+};
+
+template <int W>
+struct Interp3Mfma {
+  static hipError_t run(const PointArgs &pa, const float2 *grid, float2 *out, const Geom &g,
+                        const KbPoly &poly, int nblk, hipStream_t s) {
+    constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
+    hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
+                       grid, out, g, poly);
     return hipGetLastError();
   }
 };
@@ -735,12 +1281,24 @@ hipError_t launch_spread_xrec(int w, const PointArgs &pa, const float2 *values, 
   return dispatch_w<SpreadXrec>(w, pa, values, g, poly, rec, xo, s);
 }
 
-hipError_t launch_spread3d(int w, const PointArgs &pa, const float *rec, const uint8_t *xo,
-                           float2 *grid, const Geom &g, const KbPoly &poly, int nblk,
-                           hipStream_t s) {
+int spread_bin_x(int w, int method) {
+  int v = kSpreadBinX;
+  (void)dispatch_w<SpreadBinX>(w, method, &v);
+  return v;
+}
+
+int spread_block_threads(int w, int method) {
+  if (method != kSpreadMfma) return kSpreadThreads;
+  return w >= 8 ? 512 : 256;
+}
+
+hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
+                           const uint8_t *xo, float2 *grid, const Geom &g, const KbPoly &poly,
+                           int nblk, hipStream_t s) {
+  if (method == kSpreadMfma) return dispatch_w<Spread3Mfma>(w, pa, rec, xo, grid, g, poly, nblk, s);
   return dispatch_w<Spread3>(w, pa, rec, xo, grid, g, poly, nblk, s);
 }
-hipError_t launch_spread3d_batched(int w, const SpreadProblem *host_probs, int nprob,
+hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_probs, int nprob,
                                    SpreadProblem *dev_table, uint32_t *dev_prefix,
                                    const KbPoly &poly, int nblk, hipStream_t s) {
   for (int first = 0; first < nprob; first += kBatchChunk) {
@@ -753,12 +1311,17 @@ hipError_t launch_spread3d_batched(int w, const SpreadProblem *host_probs, int n
   hipLaunchKernelGGL(batch_prefix_kernel, dim3(1), dim3(64), 0, s, dev_table, nprob, dev_prefix);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (method == kSpreadMfma)
+    return dispatch_w<Spread3MfmaBatched>(w, (const SpreadProblem *)dev_table,
+                                          (const uint32_t *)dev_prefix, nprob, poly, nblk, s);
   return dispatch_w<Spread3Batched>(w, (const SpreadProblem *)dev_table, (const uint32_t *)dev_prefix,
                                     nprob, poly, nblk, s);
 }
 
-hipError_t launch_interp3d(int w, const PointArgs &pa, const float2 *grid, float2 *out,
-                           const Geom &g, const KbPoly &poly, int nblk, hipStream_t s) {
+hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float2 *grid,
+                           float2 *out, const Geom &g, const KbPoly &poly, int nblk,
+                           hipStream_t s) {
+  if (method == kSpreadMfma) return dispatch_w<Interp3Mfma>(w, pa, grid, out, g, poly, nblk, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
 hipError_t launch_grad3d(int w, const PointArgs &pa, const float2 *grid, const float2 *u,
@@ -781,7 +1344,12 @@ hipError_t launch_grad1d(int w, const float *x, int64_t stride, int64_t n, const
                          hipStream_t s) {
   return dispatch_w<Grad1>(w, x, stride, n, grid, u, gx, g, poly, s);
 }
-size_t spread_lds_bytes(int w, const Geom &g) {
+size_t spread_lds_bytes(int w, int method, const Geom &g) {
+  if (method == kSpreadMfma) {
+    size_t v = 0;
+    (void)dispatch_w<MSpreadLds>(w, &v);
+    return v;
+  }
   size_t v = 0;
   (void)dispatch_w<LdsSize>(w, g, 0, &v);
   return v;

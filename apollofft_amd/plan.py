@@ -32,6 +32,7 @@ def _meta_dict(m):
         "interp_max_subproblem": m.interp_max_subproblem,
         "poly_degree": m.poly_degree,
         "poly_max_err": m.poly_max_err,
+        "kernel_method": {1: "valu", 2: "mfma"}.get(m.kernel_method, m.kernel_method),
     }
 
 
@@ -51,8 +52,10 @@ class _PlanBase:
         self._ws = None
         self._ws_cap = -1
 
-    def _opts(self, grid_policy, compat, device_id, bin_size, max_subproblem):
+    def _opts(self, grid_policy, compat, device_id, bin_size, max_subproblem, kernel_method=None):
         o = N.default_opts()
+        o.kernel_method = {None: N.SPREAD_AUTO, "auto": N.SPREAD_AUTO, "valu": N.SPREAD_VALU,
+                           "mfma": N.SPREAD_MFMA}[kernel_method]
         o.grid_policy = _policy(grid_policy)
         o.compat_reference_type2_3d_scaling = int(bool(compat))
         o.device_id = -1 if device_id is None else int(device_id)
@@ -268,11 +271,11 @@ class NufftPlan3D(_PlanBase):
 
     def __init__(self, nx, ny, nz, dx, dy, dz, sigma=2, kernel_width=6,
                  grid_policy="reference", compat_reference_type2_3d_scaling=False,
-                 device_id=None, bin_size=None, max_subproblem=None):
+                 device_id=None, bin_size=None, max_subproblem=None, kernel_method=None):
         super().__init__()
         h = ctypes.c_void_p()
         o = self._opts(grid_policy, compat_reference_type2_3d_scaling, device_id, bin_size,
-                       max_subproblem)
+                       max_subproblem, kernel_method)
         N.check(N.lib().anufft_plan3d_create(int(nx), int(ny), int(nz), float(dx), float(dy),
                                              float(dz), int(sigma), int(kernel_width),
                                              ctypes.byref(o), ctypes.byref(h)), "plan3d_create")

@@ -68,6 +68,9 @@ typedef struct anufft_opts {
   int32_t device_id;                         /* -1: current device */
   int32_t bin_size[3];                       /* interpolation tile in cells; 0 = automatic */
   int32_t max_subproblem;                    /* points per subproblem; 0 = automatic */
+  int32_t kernel_method;                     /* 3D spreading and interpolation kernels:
+                                                0 automatic, 1 vector ALU (register columns,
+                                                LDS tiles), 2 FP16-split MFMA blocks */
 } anufft_opts;
 
 typedef struct anufft_meta {
@@ -87,6 +90,7 @@ typedef struct anufft_meta {
   int32_t interp_max_subproblem;
   int32_t poly_degree;  /* degree of the per-tap kernel polynomials */
   double poly_max_err;  /* max |phi - fit| on [-w, w] measured at plan creation */
+  int32_t kernel_method; /* 3D spread/interp kernels in use (1 vector ALU, 2 MFMA) */
 } anufft_meta;
 
 typedef struct anufft_plan anufft_plan;

@@ -25,12 +25,15 @@ def _t(x, dev):
     return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
 
 
-def _plan3(n, w=6, policy="reference", compat=False, bins=None, max_sub=None):
+def _plan3(n, w=6, policy="reference", compat=False, bins=None, max_sub=None, method=None):
     import apollofft_amd as A
     d = [TWO_PI / k for k in n]
     return A.NufftPlan3D(n[0], n[1], n[2], d[0], d[1], d[2], 2, w, grid_policy=policy,
                          compat_reference_type2_3d_scaling=compat, bin_size=bins,
-                         max_subproblem=max_sub), d
+                         max_subproblem=max_sub, kernel_method=method), d
+
+
+METHODS = ["valu", "mfma"]
 
 
 # ---------------------------------------------------------------- binning
@@ -70,14 +73,18 @@ def test_binning_matches_cpu_replay(dev, case):
 
 
 # ---------------------------------------------------------------- spreading
-@pytest.mark.parametrize("w", [2, 4, 6, 7, 8])
-def test_spread_fine_grid_matches_oracle(dev, oracle_mod, w):
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("w", [2, 3, 4, 5, 6, 7, 8])
+def test_spread_fine_grid_matches_oracle(dev, oracle_mod, w, method):
+    """The oversampled grid after spreading equals the FP64 restatement's
+    (reference type1_into spread loop) for both spreading kernels."""
     import apollofft_amd._native as N
     import torch
     rng = np.random.default_rng(w)
     n = (12, 10, 16)
-    plan, d = _plan3(n, w=w)
+    plan, d = _plan3(n, w=w, method=method)
     meta = plan.meta
+    assert meta["kernel_method"] == method
     xyz = uniform_points(rng, 3000)
     c = cnormal(rng, 3000)
     ref = oracle_mod.spread3d(xyz.astype(np.float64), c.astype(np.complex128), n, d, 2, w,
@@ -101,13 +108,15 @@ CASES_3D = [
     ((13, 7, 11), 8, "exact"),    # m = 26, 17 (max(14,17)), 22: prime radices 13, 17, 11
     ((8, 8, 8), 2, "reference"),
     ((20, 24, 18), 5, "exact"),
+    ((127, 6, 10), 6, "exact"),   # m = 254 = 2 * 127: Bluestein, 512-point FFTs
 ]
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("n,w,policy", CASES_3D)
-def test_type1_3d_vs_direct(dev, oracle_mod, n, w, policy):
+def test_type1_3d_vs_direct(dev, oracle_mod, n, w, policy, method):
     rng = np.random.default_rng(100 + w)
-    plan, d = _plan3(n, w=w, policy=policy)
+    plan, d = _plan3(n, w=w, policy=policy, method=method)
     M = 4000
     xyz = uniform_points(rng, M)
     c = cnormal(rng, M)
@@ -120,10 +129,11 @@ def test_type1_3d_vs_direct(dev, oracle_mod, n, w, policy):
     assert err < max(TOL, 2.0 * kb_err), (err, kb_err)
 
 
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("n,w,policy", CASES_3D)
-def test_type2_3d_vs_direct(dev, oracle_mod, n, w, policy):
+def test_type2_3d_vs_direct(dev, oracle_mod, n, w, policy, method):
     rng = np.random.default_rng(200 + w)
-    plan, d = _plan3(n, w=w, policy=policy)
+    plan, d = _plan3(n, w=w, policy=policy, method=method)
     M = 3000
     xyz = uniform_points(rng, M)
     f = cnormal(rng, n)
@@ -150,15 +160,53 @@ def test_type2_3d_compat_scaling(dev, oracle_mod):
     assert rel_l2(out, ref) < TOL
 
 
-def test_type1_clustered_and_tiny_subproblems(dev, oracle_mod):
+@pytest.mark.parametrize("method", METHODS)
+def test_type1_clustered_and_tiny_subproblems(dev, oracle_mod, method):
     """Clustered points force many subproblems per tile (max_subproblem=64)."""
     rng = np.random.default_rng(7)
     n = (16, 16, 16)
-    plan, d = _plan3(n, w=6, max_sub=64)
+    plan, d = _plan3(n, w=6, max_sub=64, method=method)
     xyz = clustered_points(rng, 20000, sigma=0.05)
     c = cnormal(rng, 20000)
     out = plan.type1(_t(xyz, dev), _t(c, dev)).cpu().numpy()
     ref = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d)
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("scale", [1e-30, 1e-6, 1.0, 1e8, 1e30])
+def test_type1_strength_range(dev, oracle_mod, method, scale):
+    """Strengths spanning six decades inside one problem, at overall scales
+    from 1e-30 to 1e30: the MFMA kernel's per-subproblem power-of-two
+    normalisation keeps its FP16 operands in range."""
+    rng = np.random.default_rng(11)
+    n = (16, 12, 10)
+    plan, d = _plan3(n, w=7, method=method)
+    M = 3000
+    xyz = uniform_points(rng, M)
+    mag = 10.0 ** rng.uniform(-3, 3, M)
+    c = (cnormal(rng, M).astype(np.complex128) * mag * scale).astype(np.complex64)
+    out = plan.type1(_t(xyz, dev), _t(c, dev)).cpu().numpy().astype(np.complex128)
+    ref = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d)
+    assert np.all(np.isfinite(out))
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("scale", [1e-30, 1.0, 1e30])
+def test_type2_mode_range(dev, oracle_mod, method, scale):
+    """Modes spanning six decades at overall scales 1e-30 .. 1e30: the MFMA
+    interpolation's per-tile power-of-two normalisation."""
+    rng = np.random.default_rng(12)
+    n = (16, 12, 10)
+    plan, d = _plan3(n, w=7, method=method)
+    M = 2000
+    xyz = uniform_points(rng, M)
+    f = cnormal(rng, n).astype(np.complex128) * 10.0 ** rng.uniform(-3, 3, n) * scale
+    f = f.astype(np.complex64)
+    out = plan.type2(_t(xyz, dev), _t(f, dev)).cpu().numpy().astype(np.complex128)
+    ref = oracle_mod.direct_type2_3d(f.astype(np.complex128), xyz.astype(np.float64), d)
+    assert np.all(np.isfinite(out))
     assert rel_l2(out, ref) < TOL
 
 
@@ -200,7 +248,7 @@ def test_type1_1d_c1_config(dev, oracle_mod):
     assert rel_l2(out, ref) < TOL
 
 
-@pytest.mark.parametrize("n,w", [(32, 6), (33, 4), (8, 6), (4, 6), (100, 8)])
+@pytest.mark.parametrize("n,w", [(32, 6), (33, 4), (8, 6), (4, 6), (100, 8), (37, 6), (127, 7)])
 def test_1d_type1_type2_vs_direct(dev, oracle_mod, n, w):
     import apollofft_amd as A
     rng = np.random.default_rng(n)
@@ -282,8 +330,9 @@ def test_host_profile_mismatch_and_f16(dev):
 
 
 # ---------------------------------------------------------------- batched
+@pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("mixed_w", [False, True])
-def test_batched_type1_matches_single(dev, oracle_mod, mixed_w):
+def test_batched_type1_matches_single(dev, oracle_mod, mixed_w, method):
     """Batched Type-1 over uneven problems (fused persistent spreading when
     the widths agree, back-to-back otherwise) equals per-problem Type-1 and
     the FP64 direct sum."""
@@ -296,7 +345,7 @@ def test_batched_type1_matches_single(dev, oracle_mod, mixed_w):
     for i, (n, M) in enumerate(zip(shapes, Ms)):
         w = 5 + (i % 2) if mixed_w else 6
         d = [TWO_PI / k for k in n]
-        plans.append(A.NufftPlan3D(*n, *d, 2, w, grid_policy="exact"))
+        plans.append(A.NufftPlan3D(*n, *d, 2, w, grid_policy="exact", kernel_method=method))
         xyz = clustered_points(rng, M, sigma=0.2) if M > 100 else uniform_points(rng, M)
         c = cnormal(rng, M)
         pos.append(_t(xyz.reshape(-1, 3), dev))
