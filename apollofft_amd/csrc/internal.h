@@ -141,9 +141,9 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float2 *grid,
                            float2 *out, const Geom &g, const KbPoly &poly, int num_blocks,
                            hipStream_t s);
-hipError_t launch_grad3d(int w, const PointArgs &pa, const float2 *grid, const float2 *u,
-                         float *gx, float *gy, float *gz, const Geom &g, const KbPoly &poly,
-                         int num_blocks, hipStream_t s);
+hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
+                         const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
+                         const KbPoly &poly, int num_blocks, hipStream_t s);
 
 hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t num_points,
                            const float2 *values, float2 *grid, const Geom &g,
@@ -158,6 +158,9 @@ hipError_t launch_grad1d(int w, const float *x, int64_t stride, int64_t num_poin
 // LDS bytes of the 3D spread/interp kernels for a geometry.
 size_t spread_lds_bytes(int w, int method, const Geom &g);
 int spread_bin_x(int w, int method);        // spreading tile interior along x
+// The MFMA gradient kernel keeps two x cells per wave; at w = 8 (32-cell
+// tiles, 16 waves) it would not fit the registers, so w = 8 uses LDS tiles.
+inline bool grad_uses_mfma(int w, int method) { return method == kSpreadMfma && w < 8; }
 int spread_block_threads(int w, int method);
 size_t interp_lds_bytes(int w, const Geom &g, bool grad);
 

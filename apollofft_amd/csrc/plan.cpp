@@ -946,14 +946,17 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
   hipError_t e;
   // the derivative always uses the unnormalized (direct-sum) scaling
   if (p->dim == 3) {
-    PointArgs pa = ex.point_args(p->bg, M, coords, stride);
-    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bg.g, 3, ex.sort_buffers(p->bg), ex.s); });
+    const bool gm = grad_uses_mfma(p->w, p->kernel_method);
+    const anufft_plan::Binning &b = gm ? p->bi : p->bg;
+    PointArgs pa = ex.point_args(b, M, coords, stride);
+    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, b.g, 3, ex.sort_buffers(b), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
-      return launch_grad3d(p->w, pa, ex.grid(), (const float2 *)u, grad[0], grad[1], grad[2],
-                           p->bg.g, p->poly, ex.blocks(p->bg, M, 4), ex.s);
+      return launch_grad3d(p->w, p->kernel_method, pa, ex.grid(), (const float2 *)u, grad[0],
+                           grad[1], grad[2], b.g, p->poly,
+                           ex.blocks(b, M, gm ? 1 : 4), ex.s);
     });
   } else {
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });

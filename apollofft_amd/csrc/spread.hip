@@ -316,6 +316,16 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa, 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// Operands enter the FP16 split multiplied by 2^12 (exact), so that the low
+// halves of operands down to 2^-14 of the largest stay FP16-normal (11
+// significant bits): values in [0, 1] become [0, 4096] (FP16 max 65504).
+// Products carry 2^24, removed with the output scale.  Kernel derivatives
+// (|phi'| < 64 for w <= 8) use 2^8.
+constexpr float kOpScale = 4096.0f;
+constexpr float kOpScaleD = 256.0f;
+constexpr float kProdUnscale = 1.0f / (4096.0f * 4096.0f);
+constexpr float kProdUnscaleD = 1.0f / (4096.0f * 256.0f);
+
 template <int W>
 struct MSpread {
   static constexpr int NW = W >= 8 ? 8 : 4;    // waves per workgroup
@@ -342,12 +352,25 @@ struct MLds {
 // two 4-wave workgroups per CU for w <= 7
 static_assert(2 * MLds<16>::SIZE <= 160 * 1024, "MFMA spreading LDS");
 
+typedef float float2v __attribute__((ext_vector_type(2)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// hi = fp16(a) (round to nearest, packed pairs), lo = fp16(a - hi) (round
+// toward zero, packed pairs).  Both halves of a register are written by one
+// packed conversion: split sequences that wrote the low and high halves of a
+// register with separate half-register instructions gave sporadically wrong
+// low parts on gfx950, while packed conversions of the same values did not.
 __device__ __forceinline__ void split8(const float *a, half8 &hi, half8 &lo) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const _Float16 h = (_Float16)a[j];
-    hi[j] = h;
-    lo[j] = (_Float16)(a[j] - (float)h);
+  for (int j = 0; j < 8; j += 2) {
+    const float2v v = {a[j], a[j + 1]};
+    const half2v h = __builtin_convertvector(v, half2v);
+    typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
+    const fp16x2 l = __builtin_amdgcn_cvt_pkrtz(a[j] - (float)h[0], a[j + 1] - (float)h[1]);
+    hi[j] = h[0];
+    hi[j + 1] = h[1];
+    lo[j] = (_Float16)l[0];
+    lo[j + 1] = (_Float16)l[1];
   }
 }
 
@@ -414,7 +437,7 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
   int e = 0;
   if (mx > 0.f) (void)frexpf(mx, &e);
   e = max(-120, min(120, e));  // both scales stay finite and normal
-  const float s_in = ldexpf(1.0f, -e), s_out = ldexpf(1.0f, e);
+  const float s_in = ldexpf(1.0f, -e), s_out = ldexpf(1.0f, e) * kProdUnscale;
 
   f32x16 acc[XW][2];
 #pragma unroll
@@ -441,10 +464,10 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
         for (int i = 0; i < PX; ++i) WX[i * kMWY + p] = 0.f;
         if (p < npts) {
           const float *r = rec + (int64_t)(start + p) * RW;
-          C[p] = make_float2(r[0] * s_in, r[1] * s_in);
+          C[p] = make_float2(r[0] * s_in * kOpScale, r[1] * s_in * kOpScale);
           const int o = xo[start + p];
 #pragma unroll
-          for (int i = 0; i < T; ++i) WX[(o + i) * kMWY + p] = r[2 + i];
+          for (int i = 0; i < T; ++i) WX[(o + i) * kMWY + p] = r[2 + i] * kOpScale;
         } else {
           C[p] = make_float2(0.f, 0.f);
         }
@@ -732,7 +755,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
     int e = 0;
     if (mx > 0.f) (void)frexpf(mx, &e);
     e = max(-120, min(120, e));
-    const float s_in = ldexpf(1.0f, -e), s_out = ldexpf(1.0f, e);
+    const float s_in = ldexpf(1.0f, -e) * kOpScale, s_out = ldexpf(1.0f, e) * kProdUnscale;
     half8 arh[XW][2], arl[XW][2], aih[XW][2], ail[XW][2];
 #pragma unroll
     for (int q = 0; q < XW; ++q) {
@@ -787,6 +810,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
           float wz[8];
           *(float4 *)wz = *(const float4 *)(WZ + k * kIRow + 16 * st + 8 * h);
           *(float4 *)(wz + 4) = *(const float4 *)(WZ + k * kIRow + 16 * st + 8 * h + 4);
+#pragma unroll
+          for (int j = 0; j < 8; ++j) wz[j] *= kOpScale;
           split8(wz, bh[st], bl[st]);
         }
         // wy at the D rows of this lane: y = 8 (r / 4) + 4 h + r % 4
@@ -838,6 +863,253 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
           v.y += u.y;
         }
         out[JIDX[p]] = make_float2(v.x * s_out, v.y * s_out);
+      }
+    }
+  }
+}
+
+// ---- MFMA coordinate gradient --------------------------------------------
+//
+// With D_i[y][k] = sum_z G_i[y][z] wz_k(z) and E_i[y][k] = sum_z G_i[y][z]
+// wz'_k(z) (two MFMA products per x cell), the three derivative sums are
+//   Sx_k = sum_i wx'_k(i) sum_y wy_k(y) D_i,  Sy_k = sum_i wx_k(i) sum_y wy'_k(y) D_i,
+//   Sz_k = sum_i wx_k(i) sum_y wy_k(y) E_i,
+// and grad_a = -scale_a Re(conj(u_k) S_a).  Two x cells per wave (the
+// second B operand and the derivative rows need the registers).
+template <int W>
+struct MGrad {
+  static constexpr int PX = MSpread<W>::PX;
+  static constexpr int XW = 2;
+  static constexpr int NW = PX / XW;
+  static constexpr int NT = 64 * NW;
+};
+constexpr int kGNP = 64;  // points staged per round (2 MFMA groups of 32)
+
+template <int NW>
+struct GLds {
+  static constexpr int ROWS = 0;                                   // float [6][kGNP][kIRow]
+  static constexpr int RED = ROWS + 6 * kGNP * kIRow * 4;          // float [NW][6][kGNP]
+  static constexpr int JIDX = RED + NW * 6 * kGNP * 4;             // int [kGNP]
+  static constexpr int PC = JIDX + kGNP * 4;                       // float [2][kPolyCoefs][kMaxTaps]
+  static constexpr int TMX = PC + 2 * kPolyCoefs * kMaxTaps * 4;   // float [16]
+  static constexpr int SIZE = TMX + 16 * 4;
+};
+
+// Values and derivatives of one axis into zero-filled rows.
+template <int W>
+__device__ __forceinline__ void stage_axis_deriv(float *row, float *drow, int l, double t, int cell,
+                                                 const float *pc, const float *pdc) {
+#pragma unroll
+  for (int i = 0; i < 32; i += 4) {
+    *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+    *(float4 *)(drow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const float z = (float)(1.0 - 2.0 * (t - (double)cell));
+  put_taps<W>(row, 1, l, z, pc);
+  put_taps<W>(drow, 1, l, z, pdc);
+}
+
+template <int W>
+__global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(PointArgs pa,
+                                                                      const float2 *grid,
+                                                                      const float2 *u, float *gx,
+                                                                      float *gy, float *gz, Geom g,
+                                                                      KbPoly poly) {
+  using MG = MGrad<W>;
+  constexpr int NT = MG::NT, XW = MG::XW, NW = MG::NW;
+  using L = GLds<NW>;
+  extern __shared__ __align__(16) char shm[];
+  float *ROWS = (float *)(shm + L::ROWS);  // WX, WY, WZ, DWX, DWY, DWZ
+  float *RED = (float *)(shm + L::RED);
+  int *JIDX = (int *)(shm + L::JIDX);
+  float *pc = (float *)(shm + L::PC);
+  float *pdc = pc + kPolyCoefs * kMaxTaps;
+  float *tmx = (float *)(shm + L::TMX);
+  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += NT) {
+    pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
+    pdc[i] = poly.dc[i / kMaxTaps][i % kMaxTaps];
+  }
+  auto row = [&](int which, int p) { return ROWS + (which * kGNP + p) * kIRow; };
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  const uint32_t nsub = *pa.nsub;
+  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x) {
+    const int4 sub = pa.subs[sidx];
+    const int bin = sub.x;
+    const int bz = bin % g.nbin[2];
+    const int by = (bin / g.nbin[2]) % g.nbin[1];
+    const int bx = bin / (g.nbin[2] * g.nbin[1]);
+    const int bx0 = bx * g.bin[0];
+    const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
+
+    float mx = 0.f;
+    const bool yok = l32 < g.pad[1];
+    const int gyy = wrap(oy + l32, g.m[1]);
+#pragma unroll
+    for (int q = 0; q < XW; ++q) {
+      const int gxx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+      const float2 *rowp = grid + ((int64_t)gxx * g.m[1] + gyy) * g.m[2];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int z = (j < 8 ? 0 : 8) + 8 * h + j;
+        if (yok && z < g.pad[2]) {
+          const float2 v = rowp[wrap(oz + z, g.m[2])];
+          mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+      }
+    }
+    mx = wave_max(mx);
+    __syncthreads();  // previous subproblem's readers of tmx / RED are done
+    if (lane == 0) tmx[wave] = mx;
+    __syncthreads();
+    mx = tmx[0];
+#pragma unroll
+    for (int i = 1; i < NW; ++i) mx = fmaxf(mx, tmx[i]);
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);
+    e = max(-120, min(120, e));
+    const float s_in = ldexpf(1.0f, -e) * kOpScale, s_out = ldexpf(1.0f, e);
+    half8 arh[XW][2], arl[XW][2], aih[XW][2], ail[XW][2];
+#pragma unroll
+    for (int q = 0; q < XW; ++q) {
+      const int gxx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+      const float2 *rowp = grid + ((int64_t)gxx * g.m[1] + gyy) * g.m[2];
+#pragma unroll
+      for (int st = 0; st < 2; ++st) {
+        float tr[8], ti[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int z = 16 * st + 8 * h + j;
+          float2 v = make_float2(0.f, 0.f);
+          if (yok && z < g.pad[2]) v = rowp[wrap(oz + z, g.m[2])];
+          tr[j] = v.x * s_in;
+          ti[j] = v.y * s_in;
+        }
+        split8(tr, arh[q][st], arl[q][st]);
+        split8(ti, aih[q][st], ail[q][st]);
+      }
+    }
+
+    for (int c0 = 0; c0 < sub.z; c0 += kGNP) {
+      const int npts = min(kGNP, sub.z - c0);
+      const int ng = (npts + 31) >> 5;
+      const int np32 = ng << 5;
+      const int start = sub.y + c0;
+      __syncthreads();
+      for (int t = threadIdx.x; t < 3 * np32; t += NT) {
+        const int ax = t / np32;
+        const int p = t - ax * np32;
+        float *r0 = row(ax, p), *r1 = row(3 + ax, p);
+        if (p < npts) {
+          const uint32_t j = pa.perm[start + p];
+          const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
+          int cell;
+          const double tt = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
+          const int l = ax == 0 ? cell - bx0 : cell - W + 1 - (ax == 1 ? oy : oz);
+          stage_axis_deriv<W>(r0, r1, l, tt, cell, pc, pdc);
+          if (ax == 0) JIDX[p] = (int)j;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 32; i += 4) {
+            *(float4 *)(r0 + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+            *(float4 *)(r1 + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+        }
+      }
+      __syncthreads();
+      for (int gi = 0; gi < ng; ++gi) {
+        const int k = (gi << 5) + l32;
+        float wy[16];
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4)
+          *(float4 *)(wy + 4 * r4) = *(const float4 *)(row(1, k) + 8 * r4 + 4 * h);
+        float sxr = 0.f, sxi = 0.f, syr = 0.f, syi = 0.f, szr = 0.f, szi = 0.f;
+        // phase 0: B = wz (D, for Sx and Sy); phase 1: B = wz' (E, for Sz)
+#pragma unroll 1
+        for (int ph = 0; ph < 2; ++ph) {
+          half8 bh[2], bl[2];
+#pragma unroll
+          for (int st = 0; st < 2; ++st) {
+            float wz[8];
+            const float *src = row(ph == 0 ? 2 : 5, k) + 16 * st + 8 * h;
+            *(float4 *)wz = *(const float4 *)src;
+            *(float4 *)(wz + 4) = *(const float4 *)(src + 4);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) wz[j] *= ph == 0 ? kOpScale : kOpScaleD;
+            split8(wz, bh[st], bl[st]);
+          }
+          float dy[16];
+          if (ph == 0) {
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4)
+              *(float4 *)(dy + 4 * r4) = *(const float4 *)(row(4, k) + 8 * r4 + 4 * h);
+          }
+#pragma unroll
+          for (int q = 0; q < XW; ++q) {
+            const float wx = row(0, k)[wave * XW + q];
+            f32x16 dr, di;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              dr[r] = 0.f;
+              di[r] = 0.f;
+            }
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {
+              dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bh[st], dr, 0, 0, 0);
+              dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bl[st], dr, 0, 0, 0);
+              dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl[q][st], bh[st], dr, 0, 0, 0);
+              di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bh[st], di, 0, 0, 0);
+              di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bl[st], di, 0, 0, 0);
+              di = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail[q][st], bh[st], di, 0, 0, 0);
+            }
+            float ar = 0.f, ai = 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+              ar = fmaf(wy[r], dr[r], ar);
+              ai = fmaf(wy[r], di[r], ai);
+            }
+            if (ph == 0) {
+              const float dwx = row(3, k)[wave * XW + q];
+              float br = 0.f, bi = 0.f;
+#pragma unroll
+              for (int r = 0; r < 16; ++r) {
+                br = fmaf(dy[r], dr[r], br);
+                bi = fmaf(dy[r], di[r], bi);
+              }
+              sxr = fmaf(dwx, ar, sxr);
+              sxi = fmaf(dwx, ai, sxi);
+              syr = fmaf(wx, br, syr);
+              syi = fmaf(wx, bi, syi);
+            } else {
+              szr = fmaf(wx, ar, szr);
+              szi = fmaf(wx, ai, szi);
+            }
+          }
+        }
+        float sv[6] = {sxr, sxi, syr, syi, szr, szi};
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          sv[c] += __shfl_xor(sv[c], 32);
+          if (h == 0) RED[(wave * 6 + c) * kGNP + k] = sv[c];
+        }
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < npts; p += NT) {
+        float sv[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          float a = 0.f;
+#pragma unroll
+          for (int w2 = 0; w2 < NW; ++w2) a += RED[(w2 * 6 + c) * kGNP + p];
+          sv[c] = a * s_out * (c < 4 ? kProdUnscale : kProdUnscaleD);
+        }
+        const int j = JIDX[p];
+        const float2 uj = u[j];
+        // Re(conj(u) * (-s) * S) = -s (u.x S.x + u.y S.y)
+        gx[j] = -(float)g.scale[0] * (uj.x * sv[0] + uj.y * sv[1]);
+        if (gy) gy[j] = -(float)g.scale[1] * (uj.x * sv[2] + uj.y * sv[3]);
+        if (gz) gz[j] = -(float)g.scale[2] * (uj.x * sv[4] + uj.y * sv[5]);
       }
     }
   }
@@ -1212,6 +1484,25 @@ struct Interp3 {
 };
 
 template <int W>
+struct Grad3Mfma {
+  static hipError_t run(const PointArgs &pa, const float2 *grid, const float2 *u, float *gx,
+                        float *gy, float *gz, const Geom &g, const KbPoly &poly, int nblk,
+                        hipStream_t s) {
+    if constexpr (W < 8) {
+      constexpr int lds = GLds<MGrad<W>::NW>::SIZE;
+      hipError_t e = hipFuncSetAttribute((const void *)grad3d_mfma_kernel<W>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(grad3d_mfma_kernel<W>, dim3(nblk), dim3(MGrad<W>::NT), lds, s, pa, grid,
+                         u, gx, gy, gz, g, poly);
+      return hipGetLastError();
+    } else {
+      return hipErrorInvalidValue;  // w = 8 uses the LDS-tile kernel (grad_uses_mfma)
+    }
+  }
+};
+
+template <int W>
 struct Grad3 {
   static hipError_t run(const PointArgs &pa, const float2 *grid, const float2 *u, float *gx,
                         float *gy, float *gz, const Geom &g, const KbPoly &poly, int nblk,
@@ -1324,9 +1615,11 @@ hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float2 
   if (method == kSpreadMfma) return dispatch_w<Interp3Mfma>(w, pa, grid, out, g, poly, nblk, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
-hipError_t launch_grad3d(int w, const PointArgs &pa, const float2 *grid, const float2 *u,
-                         float *gx, float *gy, float *gz, const Geom &g, const KbPoly &poly,
-                         int nblk, hipStream_t s) {
+hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
+                         const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
+                         const KbPoly &poly, int nblk, hipStream_t s) {
+  if (grad_uses_mfma(w, method))
+    return dispatch_w<Grad3Mfma>(w, pa, grid, u, gx, gy, gz, g, poly, nblk, s);
   return dispatch_w<Grad3>(w, pa, grid, u, gx, gy, gz, g, poly, nblk, s);
 }
 hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t n,

@@ -269,12 +269,15 @@ def test_1d_type1_type2_vs_direct(dev, oracle_mod, n, w):
 
 
 # ---------------------------------------------------------------- gradient
-def test_grad_points_type2_vs_direct(dev, oracle_mod):
-    """d/dx of L = ||T2(f,x) - b||^2 against the closed-form direct sum."""
+@pytest.mark.parametrize("method", METHODS)
+@pytest.mark.parametrize("w", [4, 7, 8])
+def test_grad_points_type2_vs_direct(dev, oracle_mod, method, w):
+    """d/dx of L = ||T2(f,x) - b||^2 against the closed-form direct sum
+    (w = 8 runs the LDS-tile gradient kernel under both methods)."""
     import torch
     rng = np.random.default_rng(3)
     n = (12, 12, 12)
-    plan, d = _plan3(n, w=7)
+    plan, d = _plan3(n, w=w, method=method)
     M = 1500
     xyz = uniform_points(rng, M)
     f = cnormal(rng, n)
