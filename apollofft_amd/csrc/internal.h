@@ -101,12 +101,13 @@ struct PointArgs {
   const uint32_t *perm;      // sorted order -> original point index
   const int4 *subs;
   const uint32_t *nsub;      // device count of subproblems
+  float *sx, *sy, *sz;       // coordinates in sorted order (filled by the prep kernels)
 };
 
 // One problem of a batched spreading launch.
 struct SpreadProblem {
   PointArgs pa;
-  const float *rec;    // per-point spreading records (launch_spread_xrec)
+  const float *rec;    // per-point records (launch_point_prep)
   const uint8_t *xo;   // per-point x offsets in the tile
   float2 *grid;
   Geom g;
@@ -127,20 +128,24 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
 
 hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
 
-// Spreading records: spread_rec_floats(w) floats and one x-offset byte per
-// sorted point, filled by launch_spread_xrec and read by launch_spread3d.
-int spread_rec_floats(int w);
-hipError_t launch_spread_xrec(int w, const PointArgs &pa, const float2 *values, const Geom &g,
-                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
+// Per-point records in sorted order: point_rec_floats(w) floats (value and
+// the tap weights of the three axes with their tile offsets) and one x-offset
+// byte per point, filled by launch_point_prep (values may be NULL for
+// Type-2) and read by the spreading and MFMA interpolation kernels.
+int point_rec_floats(int w);
+hipError_t launch_point_prep(int w, const PointArgs &pa, const float2 *values, const Geom &g,
+                             const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
+// Coordinates in sorted order for the MFMA interpolation / gradient kernels.
+hipError_t launch_gather_sorted(const PointArgs &pa, hipStream_t s);
 hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
                            const uint8_t *xo, float2 *grid, const Geom &g, const KbPoly &poly,
                            int num_blocks, hipStream_t s);
 hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_probs, int nprob,
                                    SpreadProblem *dev_table, uint32_t *dev_prefix,
                                    const KbPoly &poly, int num_blocks, hipStream_t s);
-hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float2 *grid,
-                           float2 *out, const Geom &g, const KbPoly &poly, int num_blocks,
-                           hipStream_t s);
+hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float *rec,
+                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+                           int num_blocks, hipStream_t s);
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
                          const KbPoly &poly, int num_blocks, hipStream_t s);

@@ -182,7 +182,7 @@ struct StageTimer {
 
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
-         sub_ofs = 0, subs = 0, rec = 0, xo = 0, grid = 0, fftbuf = 0, total = 0;
+         sub_ofs = 0, subs = 0, rec = 0, xo = 0, srt = 0, grid = 0, fftbuf = 0, total = 0;
   int nchunks = 0;
 };
 
@@ -265,8 +265,9 @@ Layout layout(const anufft_plan *p, int64_t M) {
     for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
       nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
     l.subs = take(16 * (size_t)nsub);
-    l.rec = take(sizeof(float) * (size_t)spread_rec_floats(p->w) * Mp);
+    l.rec = take(sizeof(float) * (size_t)point_rec_floats(p->w) * Mp);
     l.xo = take(Mp);
+    l.srt = take(3 * 4 * Mp);
     l.grid = take(8 * (size_t)p->cells());
     l.fftbuf = take(8 * (size_t)(p->m[0] * p->m[1] * p->n[2]));
   } else {
@@ -622,7 +623,7 @@ struct Exec {
   // Type-1 spreading of sorted points: records, then the tile kernel.
   hipError_t spread3(const PointArgs &pa, const float2 *values, float2 *grid) const {
     hipError_t e = timed(ANUFFT_STAGE_PREP, [&] {
-      return launch_spread_xrec(p->w, pa, values, p->bs.g, p->poly, rec(), xo(), s);
+      return launch_point_prep(p->w, pa, values, p->bs.g, p->poly, rec(), xo(), s);
     });
     if (e != hipSuccess) return e;
     return timed(ANUFFT_STAGE_SPREAD, [&] {
@@ -660,6 +661,12 @@ struct Exec {
     pa.perm = sb.idx[b.passes & 1];
     pa.subs = sb.subs;
     pa.nsub = sb.sub_ofs + b.nbins;
+    if (lay.srt) {
+      const size_t Mp = (size_t)std::max<int64_t>(M, 1);
+      pa.sx = (float *)(ws + lay.srt);
+      pa.sy = pa.sx + Mp;
+      pa.sz = pa.sy + Mp;
+    }
     return pa;
   }
   int blocks(const anufft_plan::Binning &b, int64_t M, int per_cu) const {
@@ -910,12 +917,18 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     PointArgs pa = ex.point_args(p->bi, M, coords, stride);
     e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bi.g, 3, ex.sort_buffers(p->bi), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
+    if (p->kernel_method == kSpreadMfma) {
+      e = ex.timed(ANUFFT_STAGE_PREP, [&] {
+        return launch_point_prep(p->w, pa, nullptr, p->bi.g, p->poly, ex.rec(), ex.xo(), ex.s);
+      });
+      if (e != hipSuccess) return hip_fail(e, "point records");
+    }
     e = ex.timed(ANUFFT_STAGE_FFT, [&] {
       return ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
     });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp3d(p->w, p->kernel_method, pa, ex.grid(), (float2 *)out, p->bi.g,
+      return launch_interp3d(p->w, p->kernel_method, pa, ex.rec(), ex.grid(), (float2 *)out, p->bi.g,
                              p->poly, ex.blocks(p->bi, M, p->kernel_method == kSpreadMfma ? 2 : 4),
                              ex.s);
     });
@@ -951,6 +964,10 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
     PointArgs pa = ex.point_args(b, M, coords, stride);
     e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, b.g, 3, ex.sort_buffers(b), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
+    if (gm) {
+      e = ex.timed(ANUFFT_STAGE_PREP, [&] { return launch_gather_sorted(pa, ex.s); });
+      if (e != hipSuccess) return hip_fail(e, "sorted coordinates");
+    }
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
@@ -1155,7 +1172,7 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   if (e != hipSuccess) return hip_fail(e, "binning");
   e = e0.timed(ANUFFT_STAGE_PREP, [&] {
     for (int i = 0; i < nprob; ++i) {
-      hipError_t r = launch_spread_xrec(p0->w, probs[i].pa, (const float2 *)values[i],
+      hipError_t r = launch_point_prep(p0->w, probs[i].pa, (const float2 *)values[i],
                                         plans[i]->bs.g, plans[i]->poly, ex[i].rec(), ex[i].xo(), s);
       if (r != hipSuccess) return r;
     }

@@ -105,40 +105,73 @@ __device__ __forceinline__ void stage_points(const PointArgs &pa, const Geom &g,
   }
 }
 
-// Per-point spreading record, in sorted order: the value (re, im) and the 2w
-// x-weights, zero-padded to a multiple of 4 floats.  The spreading kernel
-// reads a record with wave-uniform scalar loads, so the x-weights reach the
-// FMAs as scalar operands instead of 64-lane LDS broadcasts.
+// Per-point record, in sorted order (floats):
+//   [0, 2)          the value (re, im); zero for Type-2
+//   [2, 2 + 2w)     x tap weights,  [2 + 2w, 2 + 4w) y,  [2 + 4w, 2 + 6w) z
+//   2 + 6w          tile offsets of the first taps (bit pattern of an int:
+//                   x register cell | y column << 8 | z column << 16)
+// zero-padded to a multiple of 4.  Computing every weight once, in one fully
+// parallel pass, leaves the tile kernels only coalesced copies to stage.  The
+// vector-ALU spreading kernel reads the value and x weights of a record with
+// wave-uniform scalar loads, so they reach its FMAs as scalar operands.
 template <int W>
 struct XRec {
   static constexpr int T = 2 * W;
-  static constexpr int RW = (T + 2 + 3) / 4 * 4;  // floats per record
+  static constexpr int WX = 2, WY = 2 + T, WZ = 2 + 2 * T, OFF = 2 + 3 * T;
+  static constexpr int RW = (OFF + 1 + 3) / 4 * 4;  // floats per record
 };
 
-// Fill the records and the x offset cell - bin_x floor(cell / bin_x) of the
-// first tap's register cell, for every sorted position.
+// Fill the record and the x offset (cell mod bin_x) of every sorted point.
+// Offsets: x register cell of the first tap = cell_x mod bin_x; y (z) tile
+// column of the first tap = cell mod bin + 1 (the tile starts w cells before
+// its bin).
 template <int W>
-__global__ __launch_bounds__(256) void spread_xrec_kernel(PointArgs pa, const float2 *values,
-                                                           Geom g, KbPoly poly, float4 *rec,
-                                                           uint8_t *xo) {
-  constexpr int T = 2 * W;
-  constexpr int RW = XRec<W>::RW;
+__global__ __launch_bounds__(256) void point_prep_kernel(PointArgs pa, const float2 *values,
+                                                          Geom g, KbPoly poly, float4 *rec,
+                                                          uint8_t *xo) {
+  using R = XRec<W>;
+  constexpr int RW = R::RW;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
   for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < pa.num_points; s += st) {
-    const uint32_t j = pa.perm[s];
-    int cell;
-    const double t = fold_coord(pa.x[(int64_t)j * pa.stride], g.scale[0], g.m[0], cell);
+    const int64_t j = (int64_t)pa.perm[s];
     float r[RW];
-    kb_taps<W>(t, cell, poly, r + 2);
-    const float2 c = values[j];
-    r[0] = c.x;
-    r[1] = c.y;
+    int off = 0;
 #pragma unroll
-    for (int i = T + 2; i < RW; ++i) r[i] = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      const float *cp = a == 0 ? pa.x : (a == 1 ? pa.y : pa.z);
+      int cell;
+      const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], cell);
+      kb_taps<W>(t, cell, poly, r + R::WX + a * 2 * W);
+      const int o = cell - (cell / g.bin[a]) * g.bin[a];
+      off |= (a == 0 ? o : o + 1) << (8 * a);
+      if (a == 0) xo[s] = (uint8_t)o;
+    }
+    if (values) {
+      const float2 c = values[j];
+      r[0] = c.x;
+      r[1] = c.y;
+    } else {
+      r[0] = 0.f;
+      r[1] = 0.f;
+    }
+    r[R::OFF] = __int_as_float(off);
+#pragma unroll
+    for (int i = R::OFF + 1; i < RW; ++i) r[i] = 0.f;
     float4 *d = rec + s * (RW / 4);
 #pragma unroll
     for (int q = 0; q < RW / 4; ++q) d[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
-    xo[s] = (uint8_t)(cell - (cell / g.bin[0]) * g.bin[0]);
+  }
+}
+
+// Coordinates in sorted order (one gather through the permutation), so the
+// MFMA interpolation and gradient kernels stage points with coalesced loads.
+__global__ __launch_bounds__(256) void gather_sorted_kernel(PointArgs pa) {
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < pa.num_points; s += st) {
+    const int64_t j = (int64_t)pa.perm[s] * pa.stride;
+    pa.sx[s] = pa.x[j];
+    pa.sy[s] = pa.y[j];
+    pa.sz[s] = pa.z[j];
   }
 }
 
@@ -345,8 +378,7 @@ struct MLds {
   static constexpr int BT = WX + PX * kMWY * 4;                // half [4][32][kMBT]
   static constexpr int C = BT + 4 * 32 * kMBT * 2;             // float2 [kMNP]
   static constexpr int WZ = C + kMNP * 8;                      // float [kMNP][32]
-  static constexpr int PC = WZ + kMNP * 32 * 4;                // float [kPolyCoefs][kMaxTaps]
-  static constexpr int RED = PC + kPolyCoefs * kMaxTaps * 4;   // float [8]
+  static constexpr int RED = WZ + kMNP * 32 * 4;               // float [8]
   static constexpr int SIZE = RED + 8 * 4;
 };
 // two 4-wave workgroups per CU for w <= 7
@@ -408,7 +440,6 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
   _Float16 *BT = (_Float16 *)(sh + L::BT);
   float2 *C = (float2 *)(sh + L::C);
   float *WZ = (float *)(sh + L::WZ);
-  const float *pc = (const float *)(sh + L::PC);
   float *red = (float *)(sh + L::RED);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -455,19 +486,20 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
     const int np16 = ng << 4;
     const int start = sub.y + c0;
     __syncthreads();  // the previous round's operand reads are done
-    // pass 1: x weights and strengths (from the records), y and z weights
+    // pass 1: copy the records' strengths and weights into the operand tables
     for (int t = threadIdx.x; t < 3 * np16; t += NT) {
       const int ax = t / np16;
       const int p = t - ax * np16;
+      const float *r = rec + (int64_t)(start + p) * RW;
+      const int off = p < npts ? __float_as_int(r[XRec<W>::OFF]) : 0;
       if (ax == 0) {
 #pragma unroll
         for (int i = 0; i < PX; ++i) WX[i * kMWY + p] = 0.f;
         if (p < npts) {
-          const float *r = rec + (int64_t)(start + p) * RW;
           C[p] = make_float2(r[0] * s_in * kOpScale, r[1] * s_in * kOpScale);
-          const int o = xo[start + p];
+          const int o = off & 255;
 #pragma unroll
-          for (int i = 0; i < T; ++i) WX[(o + i) * kMWY + p] = r[2 + i] * kOpScale;
+          for (int i = 0; i < T; ++i) WX[(o + i) * kMWY + p] = r[XRec<W>::WX + i] * kOpScale;
         } else {
           C[p] = make_float2(0.f, 0.f);
         }
@@ -475,22 +507,18 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
 #pragma unroll
         for (int y = 0; y < 32; ++y) WY[y * kMWY + p] = 0.f;
         if (p < npts) {
-          const uint32_t j = pa.perm[start + p];
-          int cell;
-          const double tt = fold_coord(pa.y[(int64_t)j * pa.stride], g.scale[1], g.m[1], cell);
-          const float z = (float)(1.0 - 2.0 * (tt - (double)cell));
-          put_taps<W>(WY + p, kMWY, cell - W + 1 - oy, z, pc);
+          const int l = (off >> 8) & 255;
+#pragma unroll
+          for (int i = 0; i < T; ++i) WY[(l + i) * kMWY + p] = r[XRec<W>::WY + i];
         }
       } else {
         float *row = WZ + p * 32;
 #pragma unroll
         for (int zz = 0; zz < 32; zz += 4) *(float4 *)(row + zz) = make_float4(0.f, 0.f, 0.f, 0.f);
         if (p < npts) {
-          const uint32_t j = pa.perm[start + p];
-          int cell;
-          const double tt = fold_coord(pa.z[(int64_t)j * pa.stride], g.scale[2], g.m[2], cell);
-          const float z = (float)(1.0 - 2.0 * (tt - (double)cell));
-          put_taps<W>(row, 1, cell - W + 1 - oz, z, pc);
+          const int l = (off >> 16) & 255;
+#pragma unroll
+          for (int i = 0; i < T; ++i) row[l + i] = r[XRec<W>::WZ + i];
         }
       }
     }
@@ -569,17 +597,9 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
 }
 
 template <int W>
-__device__ __forceinline__ void mspread_load_poly(const KbPoly &poly, char *sh) {
-  float *pc = (float *)(sh + MLds<MSpread<W>::PX>::PC);
-  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += blockDim.x)
-    pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
-}
-
-template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
     PointArgs pa, const float *rec, const uint8_t *xo, float2 *grid, Geom g, KbPoly poly) {
   extern __shared__ __align__(16) char shm[];
-  mspread_load_poly<W>(poly, shm);  // ordered before use by the staging barriers
   const uint32_t nsub = *pa.nsub;
   for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x)
     spread_subproblem_mfma<W>(pa, rec, xo, grid, g, pa.subs[sidx], shm);
@@ -589,7 +609,6 @@ template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kernel(
     const SpreadProblem *probs, const uint32_t *prefix, int nprob, KbPoly poly) {
   extern __shared__ __align__(16) char shm[];
-  mspread_load_poly<W>(poly, shm);
   const uint32_t total = prefix[nprob];
   for (uint32_t gidx = blockIdx.x; gidx < total; gidx += gridDim.x) {
     int lo = 0, hi = nprob - 1;
@@ -678,8 +697,7 @@ struct ILds {
   static constexpr int WX = WY + kINP * kIRow * 4;             // float [kINP][kIRow]
   static constexpr int RED = WX + kINP * kIRow * 4;            // float2 [8][kINP]
   static constexpr int JIDX = RED + 8 * kINP * 8;              // int [kINP]
-  static constexpr int PC = JIDX + kINP * 4;                   // float [kPolyCoefs][kMaxTaps]
-  static constexpr int TMX = PC + kPolyCoefs * kMaxTaps * 4;   // float [8]
+  static constexpr int TMX = JIDX + kINP * 4;                  // float [8]
   static constexpr int SIZE = TMX + 8 * 4;
 };
 static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
@@ -695,9 +713,9 @@ __device__ __forceinline__ void stage_axis(float *row, int l, double t, int cell
 
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointArgs pa,
+                                                                          const float *rec,
                                                                           const float2 *grid,
-                                                                          float2 *out, Geom g,
-                                                                          KbPoly poly) {
+                                                                          float2 *out, Geom g) {
   using MS = MSpread<W>;
   constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW;
   using L = ILds<PX>;
@@ -707,9 +725,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
   float *WX = (float *)(shm + L::WX);
   float2 *RED = (float2 *)(shm + L::RED);
   int *JIDX = (int *)(shm + L::JIDX);
-  float *pc = (float *)(shm + L::PC);
   float *tmx = (float *)(shm + L::TMX);
-  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += NT) pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
@@ -788,14 +804,14 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
         const int p = t - ax * np32;
         float *row = (ax == 0 ? WX : (ax == 1 ? WY : WZ)) + p * kIRow;
         if (p < npts) {
-          const uint32_t j = pa.perm[start + p];
-          const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
-          int cell;
-          const double tt = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
-          // x: register cell of the first tap; y, z: tile column of the first tap
-          const int l = ax == 0 ? cell - bx0 : cell - W + 1 - (ax == 1 ? oy : oz);
-          stage_axis<W>(row, l, tt, cell, pc);
-          if (ax == 0) JIDX[p] = (int)j;
+          // copy the record's weights of this axis to its first tap's cell
+          const float *r = rec + (int64_t)(start + p) * XRec<W>::RW;
+          const int l = (__float_as_int(r[XRec<W>::OFF]) >> (8 * ax)) & 255;
+#pragma unroll
+          for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int i = 0; i < 2 * W; ++i) row[l + i] = r[XRec<W>::WX + ax * 2 * W + i];
+          if (ax == 0) JIDX[p] = (int)pa.perm[start + p];
         } else {
 #pragma unroll
           for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1002,13 +1018,12 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(PointArgs 
         const int p = t - ax * np32;
         float *r0 = row(ax, p), *r1 = row(3 + ax, p);
         if (p < npts) {
-          const uint32_t j = pa.perm[start + p];
-          const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
+          const float *cp = ax == 0 ? pa.sx : (ax == 1 ? pa.sy : pa.sz);
           int cell;
-          const double tt = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
+          const double tt = fold_coord(cp[start + p], g.scale[ax], g.m[ax], cell);
           const int l = ax == 0 ? cell - bx0 : cell - W + 1 - (ax == 1 ? oy : oz);
           stage_axis_deriv<W>(r0, r1, l, tt, cell, pc, pdc);
-          if (ax == 0) JIDX[p] = (int)j;
+          if (ax == 0) JIDX[p] = (int)pa.perm[start + p];
         } else {
 #pragma unroll
           for (int i = 0; i < 32; i += 4) {
@@ -1370,13 +1385,13 @@ hipError_t dispatch_w(int w, A &&...args) {
 }
 
 template <int W>
-struct SpreadXrec {
+struct PointPrep {
   static hipError_t run(const PointArgs &pa, const float2 *values, const Geom &g,
                         const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s) {
     if (pa.num_points <= 0) return hipSuccess;
     int64_t blocks = (pa.num_points + 255) / 256;
     if (blocks > 16384) blocks = 16384;
-    hipLaunchKernelGGL(spread_xrec_kernel<W>, dim3((unsigned)blocks), dim3(256), 0, s, pa, values,
+    hipLaunchKernelGGL(point_prep_kernel<W>, dim3((unsigned)blocks), dim3(256), 0, s, pa, values,
                        g, poly, (float4 *)rec, xo);
     return hipGetLastError();
   }
@@ -1457,14 +1472,14 @@ struct Spread3Batched {
 
 template <int W>
 struct Interp3Mfma {
-  static hipError_t run(const PointArgs &pa, const float2 *grid, float2 *out, const Geom &g,
-                        const KbPoly &poly, int nblk, hipStream_t s) {
+  static hipError_t run(const PointArgs &pa, const float *rec, const float2 *grid, float2 *out,
+                        const Geom &g, int nblk, hipStream_t s) {
     constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                       grid, out, g, poly);
+                       rec, grid, out, g);
     return hipGetLastError();
   }
 };
@@ -1565,11 +1580,19 @@ struct LdsSize {
 
 } // namespace
 
-int spread_rec_floats(int w) { return (2 * w + 2 + 3) / 4 * 4; }
+int point_rec_floats(int w) { return (2 + 6 * w + 1 + 3) / 4 * 4; }
 
-hipError_t launch_spread_xrec(int w, const PointArgs &pa, const float2 *values, const Geom &g,
-                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s) {
-  return dispatch_w<SpreadXrec>(w, pa, values, g, poly, rec, xo, s);
+hipError_t launch_gather_sorted(const PointArgs &pa, hipStream_t s) {
+  if (pa.num_points <= 0) return hipSuccess;
+  int64_t blocks = (pa.num_points + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(gather_sorted_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pa);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_prep(int w, const PointArgs &pa, const float2 *values, const Geom &g,
+                             const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s) {
+  return dispatch_w<PointPrep>(w, pa, values, g, poly, rec, xo, s);
 }
 
 int spread_bin_x(int w, int method) {
@@ -1609,10 +1632,10 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
                                     nprob, poly, nblk, s);
 }
 
-hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float2 *grid,
-                           float2 *out, const Geom &g, const KbPoly &poly, int nblk,
-                           hipStream_t s) {
-  if (method == kSpreadMfma) return dispatch_w<Interp3Mfma>(w, pa, grid, out, g, poly, nblk, s);
+hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float *rec,
+                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+                           int nblk, hipStream_t s) {
+  if (method == kSpreadMfma) return dispatch_w<Interp3Mfma>(w, pa, rec, grid, out, g, nblk, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
