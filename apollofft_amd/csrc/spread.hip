@@ -554,6 +554,7 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
       float wy[8];
       *(float4 *)wy = *(const float4 *)(WY + l32 * kMWY + kb);
       *(float4 *)(wy + 4) = *(const float4 *)(WY + l32 * kMWY + kb + 4);
+      half8 ah[XW], al[XW];
 #pragma unroll
       for (int q = 0; q < XW; ++q) {
         const int ix = wave * XW + q;
@@ -562,14 +563,24 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
         *(float4 *)(a + 4) = *(const float4 *)(WX + ix * kMWY + kb + 4);
 #pragma unroll
         for (int j = 0; j < 8; ++j) a[j] *= wy[j];
-        half8 ah, al;
-        split8(a, ah, al);
-        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, brh, acc[q][0], 0, 0, 0);
-        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, brl, acc[q][0], 0, 0, 0);
-        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, brh, acc[q][0], 0, 0, 0);
-        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bih, acc[q][1], 0, 0, 0);
-        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bil, acc[q][1], 0, 0, 0);
-        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bih, acc[q][1], 0, 0, 0);
+        split8(a, ah[q], al[q]);
+      }
+      // one split term at a time over all 2 XW accumulators: 2 XW
+      // independent MFMA chains
+#pragma unroll
+      for (int q = 0; q < XW; ++q) {
+        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[q], brh, acc[q][0], 0, 0, 0);
+        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[q], bih, acc[q][1], 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < XW; ++q) {
+        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[q], brl, acc[q][0], 0, 0, 0);
+        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[q], bil, acc[q][1], 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < XW; ++q) {
+        acc[q][0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[q], brh, acc[q][0], 0, 0, 0);
+        acc[q][1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[q], bih, acc[q][1], 0, 0, 0);
       }
     }
   }
