@@ -104,11 +104,23 @@ struct PointArgs {
   float *sx, *sy, *sz;       // coordinates in sorted order (filled by the prep kernels)
 };
 
+// Compact per-point records in sorted order for the MFMA kernels (SoA):
+// z[a][s] = 1 - 2 (t - cell) of axis a (the argument of the tap
+// polynomials, FP64-folded and rounded once, exactly as kb_taps computes it),
+// off[s] = x register cell | y column << 8 | z column << 16 of the first taps
+// in the point's tile, c[s] = value (Type-1 only).
+struct ZRec {
+  float *z[3];
+  int *off;
+  float2 *c;
+};
+
 // One problem of a batched spreading launch.
 struct SpreadProblem {
   PointArgs pa;
-  const float *rec;    // per-point records (launch_point_prep)
+  const float *rec;    // per-point records (launch_point_prep, vector-ALU kernel)
   const uint8_t *xo;   // per-point x offsets in the tile
+  ZRec zr;             // compact records (launch_point_z, MFMA kernel)
   float2 *grid;
   Geom g;
 };
@@ -133,18 +145,21 @@ hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
 // byte per point, filled by launch_point_prep (values may be NULL for
 // Type-2) and read by the spreading and MFMA interpolation kernels.
 int point_rec_floats(int w);
+hipError_t launch_point_z(const PointArgs &pa, const float2 *values, const Geom &g, ZRec zr,
+                          hipStream_t s);
 hipError_t launch_point_prep(int w, const PointArgs &pa, const float2 *values, const Geom &g,
                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
 // Coordinates in sorted order for the MFMA interpolation / gradient kernels.
 hipError_t launch_gather_sorted(const PointArgs &pa, hipStream_t s);
 hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
-                           const uint8_t *xo, float2 *grid, const Geom &g, const KbPoly &poly,
-                           int num_blocks, hipStream_t s);
+                           const uint8_t *xo, ZRec zr, float2 *grid, const Geom &g,
+                           const KbPoly &poly, const KbPoly *dpoly, int num_blocks, hipStream_t s);
 hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_probs, int nprob,
                                    SpreadProblem *dev_table, uint32_t *dev_prefix,
-                                   const KbPoly &poly, int num_blocks, hipStream_t s);
-hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float *rec,
-                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+                                   const KbPoly &poly, const KbPoly *dpoly, int num_blocks,
+                                   hipStream_t s);
+hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
+                           float2 *out, const Geom &g, const KbPoly &poly, const KbPoly *dpoly,
                            int num_blocks, hipStream_t s);
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,

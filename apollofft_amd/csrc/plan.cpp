@@ -182,7 +182,8 @@ struct StageTimer {
 
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
-         sub_ofs = 0, subs = 0, rec = 0, xo = 0, srt = 0, grid = 0, fftbuf = 0, total = 0;
+         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, srt = 0, grid = 0, fftbuf = 0,
+         total = 0;
   int nchunks = 0;
 };
 
@@ -206,6 +207,7 @@ struct anufft_plan {
   int blue_n[3] = {0, 0, 0};        // Bluestein length per axis (0: direct mixed radix)
   float2 *d_chirp[3] = {nullptr, nullptr, nullptr};
   float2 *d_bhat[3] = {nullptr, nullptr, nullptr};
+  KbPoly *d_poly = nullptr;         // device copy of poly (scalar-loaded by the MFMA kernels)
   KbPoly poly{};
   double poly_err = 0;
   // Binning geometries: spreading (register columns) and interpolation /
@@ -265,7 +267,10 @@ Layout layout(const anufft_plan *p, int64_t M) {
     for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
       nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
     l.subs = take(16 * (size_t)nsub);
-    l.rec = take(sizeof(float) * (size_t)point_rec_floats(p->w) * Mp);
+    if (p->kernel_method == kSpreadMfma)
+      l.zrec = take(24 * Mp);  // z[3], off (4 B each), value (8 B) per point
+    else
+      l.rec = take(sizeof(float) * (size_t)point_rec_floats(p->w) * Mp);
     l.xo = take(Mp);
     l.srt = take(3 * 4 * Mp);
     l.grid = take(8 * (size_t)p->cells());
@@ -371,6 +376,11 @@ anufft_status build_tables(anufft_plan *p) {
   if (p->device < 0) return ANUFFT_OK;
 
   DeviceGuard guard(p->device);
+  {
+    hipError_t e = hipMalloc(&p->d_poly, sizeof(KbPoly));
+    if (e == hipSuccess) e = hipMemcpy(p->d_poly, &p->poly, sizeof(KbPoly), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "device kernel polynomials");
+  }
   hipError_t es = hipStreamCreateWithFlags(&p->host_stream, hipStreamNonBlocking);
   if (es != hipSuccess) return hip_fail(es, "hipStreamCreate");
   for (int a = 0; a < p->dim; ++a) {
@@ -408,6 +418,7 @@ void destroy_plan(anufft_plan *p) {
       if (p->d_chirp[a]) (void)hipFree(p->d_chirp[a]);
       if (p->d_bhat[a]) (void)hipFree(p->d_bhat[a]);
     }
+    if (p->d_poly) (void)hipFree(p->d_poly);
     if (p->host_stream) (void)hipStreamDestroy(p->host_stream);
     for (auto &r : p->timer.recs) {
       (void)hipEventDestroy(r.a);
@@ -620,15 +631,32 @@ struct Exec {
   float2 *grid() const { return (float2 *)(ws + lay.grid); }
   float *rec() const { return (float *)(ws + lay.rec); }
   uint8_t *xo() const { return (uint8_t *)(ws + lay.xo); }
+  ZRec zrec(int64_t M) const {
+    const size_t Mp = (size_t)std::max<int64_t>(M, 1);
+    ZRec z{};
+    float *b = (float *)(ws + lay.zrec);
+    z.z[0] = b;
+    z.z[1] = b + Mp;
+    z.z[2] = b + 2 * Mp;
+    z.off = (int *)(b + 3 * Mp);
+    z.c = (float2 *)(b + 4 * Mp);
+    return z;
+  }
+  // Per-point records of sorted points: compact ones for the MFMA kernels,
+  // full weight records for the vector-ALU spreading kernel.
+  hipError_t prep(const anufft_plan::Binning &b, const PointArgs &pa, const float2 *values) const {
+    return timed(ANUFFT_STAGE_PREP, [&] {
+      if (p->kernel_method == kSpreadMfma) return launch_point_z(pa, values, b.g, zrec(pa.num_points), s);
+      return launch_point_prep(p->w, pa, values, b.g, p->poly, rec(), xo(), s);
+    });
+  }
   // Type-1 spreading of sorted points: records, then the tile kernel.
   hipError_t spread3(const PointArgs &pa, const float2 *values, float2 *grid) const {
-    hipError_t e = timed(ANUFFT_STAGE_PREP, [&] {
-      return launch_point_prep(p->w, pa, values, p->bs.g, p->poly, rec(), xo(), s);
-    });
+    hipError_t e = prep(p->bs, pa, values);
     if (e != hipSuccess) return e;
     return timed(ANUFFT_STAGE_SPREAD, [&] {
-      return launch_spread3d(p->w, p->kernel_method, pa, rec(), xo(), grid, p->bs.g, p->poly,
-                             blocks(p->bs, pa.num_points, 2), s);
+      return launch_spread3d(p->w, p->kernel_method, pa, rec(), xo(), zrec(pa.num_points), grid,
+                             p->bs.g, p->poly, p->d_poly, blocks(p->bs, pa.num_points, 2), s);
     });
   }
   float2 *fftbuf() const { return (float2 *)(ws + lay.fftbuf); }
@@ -918,9 +946,7 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bi.g, 3, ex.sort_buffers(p->bi), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
     if (p->kernel_method == kSpreadMfma) {
-      e = ex.timed(ANUFFT_STAGE_PREP, [&] {
-        return launch_point_prep(p->w, pa, nullptr, p->bi.g, p->poly, ex.rec(), ex.xo(), ex.s);
-      });
+      e = ex.prep(p->bi, pa, nullptr);
       if (e != hipSuccess) return hip_fail(e, "point records");
     }
     e = ex.timed(ANUFFT_STAGE_FFT, [&] {
@@ -928,9 +954,9 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp3d(p->w, p->kernel_method, pa, ex.rec(), ex.grid(), (float2 *)out, p->bi.g,
-                             p->poly, ex.blocks(p->bi, M, p->kernel_method == kSpreadMfma ? 2 : 4),
-                             ex.s);
+      return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(), (float2 *)out,
+                             p->bi.g, p->poly, p->d_poly,
+                             ex.blocks(p->bi, M, p->kernel_method == kSpreadMfma ? 2 : 4), ex.s);
     });
   } else {
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes, false); });
@@ -1145,6 +1171,7 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
     probs[i].pa = ex[i].point_args(plans[i]->bs, num_points[i], c3, stride);
     probs[i].rec = ex[i].rec();
     probs[i].xo = ex[i].xo();
+    probs[i].zr = ex[i].zrec(num_points[i]);
     probs[i].grid = ex[i].grid();
     probs[i].g = plans[i]->bs.g;
     sub_upper += plans[i]->bs.nbins + (num_points[i] + plans[i]->bs.max_sub - 1) / plans[i]->bs.max_sub;
@@ -1172,8 +1199,12 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   if (e != hipSuccess) return hip_fail(e, "binning");
   e = e0.timed(ANUFFT_STAGE_PREP, [&] {
     for (int i = 0; i < nprob; ++i) {
-      hipError_t r = launch_point_prep(p0->w, probs[i].pa, (const float2 *)values[i],
-                                        plans[i]->bs.g, plans[i]->poly, ex[i].rec(), ex[i].xo(), s);
+      hipError_t r = p0->kernel_method == kSpreadMfma
+                         ? launch_point_z(probs[i].pa, (const float2 *)values[i], plans[i]->bs.g,
+                                          probs[i].zr, s)
+                         : launch_point_prep(p0->w, probs[i].pa, (const float2 *)values[i],
+                                             plans[i]->bs.g, plans[i]->poly, ex[i].rec(),
+                                             ex[i].xo(), s);
       if (r != hipSuccess) return r;
     }
     return hipSuccess;
@@ -1182,7 +1213,7 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(sub_upper, 2 * (int64_t)p0->cus));
   e = e0.timed(ANUFFT_STAGE_SPREAD, [&] {
     return launch_spread3d_batched(p0->w, p0->kernel_method, probs.data(), nprob, table, prefix,
-                                   p0->poly, nblk, s);
+                                   p0->poly, p0->d_poly, nblk, s);
   });
   if (e != hipSuccess) return hip_fail(e, "spread");
   e = e0.timed(ANUFFT_STAGE_FFT, [&] {

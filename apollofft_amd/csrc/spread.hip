@@ -163,6 +163,49 @@ __global__ __launch_bounds__(256) void point_prep_kernel(PointArgs pa, const flo
   }
 }
 
+// Compact records for the MFMA kernels: per sorted point the three tap
+// polynomial arguments z = 1 - 2 (t - cell), the tile offsets of the first
+// taps and the value.
+__global__ __launch_bounds__(256) void point_z_kernel(PointArgs pa, const float2 *values, Geom g,
+                                                      ZRec zr) {
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < pa.num_points; s += st) {
+    const int64_t j = (int64_t)pa.perm[s];
+    int off = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float *cp = a == 0 ? pa.x : (a == 1 ? pa.y : pa.z);
+      int cell;
+      const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], cell);
+      zr.z[a][s] = (float)(1.0 - 2.0 * (t - (double)cell));
+      const int o = cell - (cell / g.bin[a]) * g.bin[a];
+      off |= (a == 0 ? o : o + 1) << (8 * a);
+    }
+    zr.off[s] = off;
+    if (values) zr.c[s] = values[j];
+  }
+}
+
+// The polynomial of one tap (wave-uniform index i), coefficients read once
+// with scalar loads; value at argument z with kb_taps's endpoint rule.
+template <int W>
+struct TapPoly {
+  float c[kPolyCoefs];
+  int i;
+  __device__ __forceinline__ TapPoly(const KbPoly *__restrict__ P, int tap) : i(tap) {
+#pragma unroll
+    for (int d = 0; d < kPolyCoefs; ++d) c[d] = P->c[d][tap];
+  }
+  __device__ __forceinline__ float operator()(float z) const {
+    float v = c[kPolyCoefs - 1];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, c[d]);
+    if (i == 0 && z <= -1.0f) v = 0.0f;
+    if (i == 2 * W - 1 && z >= 1.0f) v = 0.0f;
+    return v;
+  }
+};
+
 // Coordinates in sorted order (one gather through the permutation), so the
 // MFMA interpolation and gradient kernels stage points with coalesced loads.
 __global__ __launch_bounds__(256) void gather_sorted_kernel(PointArgs pa) {
@@ -371,14 +414,17 @@ constexpr int kMNP = 128;       // points staged per round
 constexpr int kMWY = kMNP + 4;  // row stride (floats) of the y / x weight tables
 constexpr int kMBT = kMNP + 8;  // row stride (halves) of the B operand tables
 // LDS layout of the MFMA spreading kernels (bytes), for a tile of PX x cells.
+constexpr int kMWZ = 33;        // row stride (floats) of the staged z weights
 template <int PX>
 struct MLds {
   static constexpr int WY = 0;                                 // float [32][kMWY]
   static constexpr int WX = WY + 32 * kMWY * 4;                // float [PX][kMWY]
   static constexpr int BT = WX + PX * kMWY * 4;                // half [4][32][kMBT]
   static constexpr int C = BT + 4 * 32 * kMBT * 2;             // float2 [kMNP]
-  static constexpr int WZ = C + kMNP * 8;                      // float [kMNP][32]
-  static constexpr int RED = WZ + kMNP * 32 * 4;               // float [8]
+  static constexpr int WZ = C + kMNP * 8;                      // float [kMNP][kMWZ]
+  static constexpr int ZS = WZ + kMNP * kMWZ * 4;              // float [3][kMNP]
+  static constexpr int OFS = ZS + 3 * kMNP * 4;                // int [kMNP]
+  static constexpr int RED = OFS + kMNP * 4;                   // float [8]
   static constexpr int SIZE = RED + 8 * 4;
 };
 // two 4-wave workgroups per CU for w <= 7
@@ -428,18 +474,19 @@ __device__ __forceinline__ void put_taps(float *dst, int step, int l, float z, c
 }
 
 template <int W>
-__device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, const float *rec,
-                                                       const uint8_t *xo, float2 *grid,
-                                                       const Geom &g, int4 sub, char *sh) {
+__device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *grid, const Geom &g,
+                                                       const KbPoly *__restrict__ dpoly, int4 sub,
+                                                       char *sh) {
   using MS = MSpread<W>;
-  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, T = 2 * W;
-  constexpr int RW = XRec<W>::RW;
+  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW, T = 2 * W;
   using L = MLds<PX>;
   float *WY = (float *)(sh + L::WY);
   float *WX = (float *)(sh + L::WX);
   _Float16 *BT = (_Float16 *)(sh + L::BT);
   float2 *C = (float2 *)(sh + L::C);
   float *WZ = (float *)(sh + L::WZ);
+  float *ZS = (float *)(sh + L::ZS);
+  int *OFS = (int *)(sh + L::OFS);
   float *red = (float *)(sh + L::RED);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -455,8 +502,8 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
   // power-of-two scale of the strengths: max |c| * 2^-e in [1/2, 1)
   float mx = 0.f;
   for (int p = threadIdx.x; p < sub.z; p += NT) {
-    const float *r = rec + (int64_t)(sub.y + p) * RW;
-    mx = fmaxf(mx, fmaxf(fabsf(r[0]), fabsf(r[1])));
+    const float2 c = zr.c[sub.y + p];
+    mx = fmaxf(mx, fmaxf(fabsf(c.x), fabsf(c.y)));
   }
   mx = wave_max(mx);
   __syncthreads();  // previous subproblem's readers of red[] are done
@@ -486,40 +533,34 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
     const int np16 = ng << 4;
     const int start = sub.y + c0;
     __syncthreads();  // the previous round's operand reads are done
-    // pass 1: copy the records' strengths and weights into the operand tables
-    for (int t = threadIdx.x; t < 3 * np16; t += NT) {
-      const int ax = t / np16;
-      const int p = t - ax * np16;
-      const float *r = rec + (int64_t)(start + p) * RW;
-      const int off = p < npts ? __float_as_int(r[XRec<W>::OFF]) : 0;
-      if (ax == 0) {
+    // pass 0: zero the operand tables; the points' polynomial arguments,
+    // tile offsets and scaled values into LDS (padding points: z = 0,
+    // offsets 0, value 0, so their finite weights meet zero B rows)
+    for (int p = threadIdx.x; p < np16; p += NT) {
 #pragma unroll
-        for (int i = 0; i < PX; ++i) WX[i * kMWY + p] = 0.f;
-        if (p < npts) {
-          C[p] = make_float2(r[0] * s_in * kOpScale, r[1] * s_in * kOpScale);
-          const int o = off & 255;
+      for (int i = 0; i < PX; ++i) WX[i * kMWY + p] = 0.f;
 #pragma unroll
-          for (int i = 0; i < T; ++i) WX[(o + i) * kMWY + p] = r[XRec<W>::WX + i] * kOpScale;
-        } else {
-          C[p] = make_float2(0.f, 0.f);
-        }
-      } else if (ax == 1) {
+      for (int y = 0; y < 32; ++y) WY[y * kMWY + p] = 0.f;
 #pragma unroll
-        for (int y = 0; y < 32; ++y) WY[y * kMWY + p] = 0.f;
-        if (p < npts) {
-          const int l = (off >> 8) & 255;
+      for (int zz = 0; zz < 32; ++zz) WZ[p * kMWZ + zz] = 0.f;
+      const bool ok = p < npts;
 #pragma unroll
-          for (int i = 0; i < T; ++i) WY[(l + i) * kMWY + p] = r[XRec<W>::WY + i];
-        }
-      } else {
-        float *row = WZ + p * 32;
-#pragma unroll
-        for (int zz = 0; zz < 32; zz += 4) *(float4 *)(row + zz) = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (p < npts) {
-          const int l = (off >> 16) & 255;
-#pragma unroll
-          for (int i = 0; i < T; ++i) row[l + i] = r[XRec<W>::WZ + i];
-        }
+      for (int a = 0; a < 3; ++a) ZS[a * kMNP + p] = ok ? zr.z[a][start + p] : 0.f;
+      OFS[p] = ok ? zr.off[start + p] : 0;
+      const float2 c = ok ? zr.c[start + p] : make_float2(0.f, 0.f);
+      C[p] = make_float2(c.x * s_in * kOpScale, c.y * s_in * kOpScale);
+    }
+    __syncthreads();
+    // pass 1: tap weights; each wave evaluates whole taps (coefficients in
+    // scalar registers, loaded once per tap) for all points and axes
+    for (int i0 = wave; i0 < T; i0 += NW) {
+      const TapPoly<W> tp(dpoly, __builtin_amdgcn_readfirstlane(i0));
+      const int i = tp.i;
+      for (int p = lane; p < np16; p += 64) {
+        const int off = OFS[p];
+        WX[((off & 255) + i) * kMWY + p] = tp(ZS[p]) * kOpScale;
+        WY[(((off >> 8) & 255) + i) * kMWY + p] = tp(ZS[kMNP + p]);
+        WZ[p * kMWZ + ((off >> 16) & 255) + i] = tp(ZS[2 * kMNP + p]);
       }
     }
     __syncthreads();
@@ -529,7 +570,7 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
       float br[8], bi[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const float w = WZ[(pb + j) * 32 + z];
+        const float w = WZ[(pb + j) * kMWZ + z];
         const float2 c = C[pb + j];
         br[j] = c.x * w;
         bi[j] = c.y * w;
@@ -609,16 +650,17 @@ __device__ __forceinline__ void spread_subproblem_mfma(const PointArgs &pa, cons
 
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
-    PointArgs pa, const float *rec, const uint8_t *xo, float2 *grid, Geom g, KbPoly poly) {
+    PointArgs pa, ZRec zr, float2 *grid, Geom g, const KbPoly *__restrict__ dpoly) {
   extern __shared__ __align__(16) char shm[];
   const uint32_t nsub = *pa.nsub;
   for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x)
-    spread_subproblem_mfma<W>(pa, rec, xo, grid, g, pa.subs[sidx], shm);
+    spread_subproblem_mfma<W>(zr, grid, g, dpoly, pa.subs[sidx], shm);
 }
 
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kernel(
-    const SpreadProblem *probs, const uint32_t *prefix, int nprob, KbPoly poly) {
+    const SpreadProblem *probs, const uint32_t *prefix, int nprob,
+    const KbPoly *__restrict__ dpoly) {
   extern __shared__ __align__(16) char shm[];
   const uint32_t total = prefix[nprob];
   for (uint32_t gidx = blockIdx.x; gidx < total; gidx += gridDim.x) {
@@ -629,8 +671,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kerne
       else hi = mid - 1;
     }
     const SpreadProblem &pr = probs[lo];
-    spread_subproblem_mfma<W>(pr.pa, pr.rec, pr.xo, pr.grid, pr.g, pr.pa.subs[gidx - prefix[lo]],
-                              shm);
+    spread_subproblem_mfma<W>(pr.zr, pr.grid, pr.g, dpoly, pr.pa.subs[gidx - prefix[lo]], shm);
   }
 }
 
@@ -708,7 +749,9 @@ struct ILds {
   static constexpr int WX = WY + kINP * kIRow * 4;             // float [kINP][kIRow]
   static constexpr int RED = WX + kINP * kIRow * 4;            // float2 [8][kINP]
   static constexpr int JIDX = RED + 8 * kINP * 8;              // int [kINP]
-  static constexpr int TMX = JIDX + kINP * 4;                  // float [8]
+  static constexpr int ZS = JIDX + kINP * 4;                   // float [3][kINP]
+  static constexpr int OFS = ZS + 3 * kINP * 4;                // int [kINP]
+  static constexpr int TMX = OFS + kINP * 4;                   // float [8]
   static constexpr int SIZE = TMX + 8 * 4;
 };
 static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
@@ -723,10 +766,9 @@ __device__ __forceinline__ void stage_axis(float *row, int l, double t, int cell
 }
 
 template <int W>
-__global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointArgs pa,
-                                                                          const float *rec,
-                                                                          const float2 *grid,
-                                                                          float2 *out, Geom g) {
+__global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
+    PointArgs pa, ZRec zr, const float2 *grid, float2 *out, Geom g,
+    const KbPoly *__restrict__ dpoly) {
   using MS = MSpread<W>;
   constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW;
   using L = ILds<PX>;
@@ -736,6 +778,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
   float *WX = (float *)(shm + L::WX);
   float2 *RED = (float2 *)(shm + L::RED);
   int *JIDX = (int *)(shm + L::JIDX);
+  float *ZS = (float *)(shm + L::ZS);
+  int *OFS = (int *)(shm + L::OFS);
   float *tmx = (float *)(shm + L::TMX);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -810,22 +854,31 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(PointA
       const int np32 = ng << 5;
       const int start = sub.y + c0;
       __syncthreads();  // the previous round's staged rows and RED are consumed
-      for (int t = threadIdx.x; t < 3 * np32; t += NT) {
-        const int ax = t / np32;
-        const int p = t - ax * np32;
-        float *row = (ax == 0 ? WX : (ax == 1 ? WY : WZ)) + p * kIRow;
-        if (p < npts) {
-          // copy the record's weights of this axis to its first tap's cell
-          const float *r = rec + (int64_t)(start + p) * XRec<W>::RW;
-          const int l = (__float_as_int(r[XRec<W>::OFF]) >> (8 * ax)) & 255;
+      // pass 0: zero the staged rows; arguments, offsets, output indices
+      for (int p = threadIdx.x; p < np32; p += NT) {
 #pragma unroll
-          for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = 0; i < 32; i += 4) {
+          *(float4 *)(WX + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+          *(float4 *)(WY + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+          *(float4 *)(WZ + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const bool ok = p < npts;
 #pragma unroll
-          for (int i = 0; i < 2 * W; ++i) row[l + i] = r[XRec<W>::WX + ax * 2 * W + i];
-          if (ax == 0) JIDX[p] = (int)pa.perm[start + p];
-        } else {
-#pragma unroll
-          for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int a = 0; a < 3; ++a) ZS[a * kINP + p] = ok ? zr.z[a][start + p] : 0.f;
+        OFS[p] = ok ? zr.off[start + p] : 0;
+        if (ok) JIDX[p] = (int)pa.perm[start + p];
+      }
+      __syncthreads();
+      // pass 1: tap weights; each wave evaluates whole taps (coefficients
+      // in scalar registers) for all points and axes
+      for (int i0 = wave; i0 < 2 * W; i0 += NW) {
+        const TapPoly<W> tp(dpoly, __builtin_amdgcn_readfirstlane(i0));
+        const int i = tp.i;
+        for (int p = lane; p < np32; p += 64) {
+          const int off = OFS[p];
+          WX[p * kIRow + (off & 255) + i] = tp(ZS[p]);
+          WY[p * kIRow + ((off >> 8) & 255) + i] = tp(ZS[kINP + p]);
+          WZ[p * kIRow + ((off >> 16) & 255) + i] = tp(ZS[2 * kINP + p]);
         }
       }
       __syncthreads();
@@ -1424,14 +1477,14 @@ struct Spread3 {
 
 template <int W>
 struct Spread3Mfma {
-  static hipError_t run(const PointArgs &pa, const float *rec, const uint8_t *xo, float2 *grid,
-                        const Geom &g, const KbPoly &poly, int nblk, hipStream_t s) {
+  static hipError_t run(const PointArgs &pa, ZRec zr, float2 *grid, const Geom &g,
+                        const KbPoly *dpoly, int nblk, hipStream_t s) {
     constexpr int lds = MLds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)spread3d_mfma_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(spread3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                       rec, xo, grid, g, poly);
+    hipLaunchKernelGGL(spread3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa, zr,
+                       grid, g, dpoly);
     return hipGetLastError();
   }
 };
@@ -1439,7 +1492,7 @@ struct Spread3Mfma {
 template <int W>
 struct Spread3MfmaBatched {
   static hipError_t run(const SpreadProblem *probs, const uint32_t *prefix, int nprob,
-                        const KbPoly &poly, int nblk, hipStream_t s) {
+                        const KbPoly *poly, int nblk, hipStream_t s) {
     constexpr int lds = MLds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)spread3d_mfma_batched_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -1483,14 +1536,14 @@ struct Spread3Batched {
 
 template <int W>
 struct Interp3Mfma {
-  static hipError_t run(const PointArgs &pa, const float *rec, const float2 *grid, float2 *out,
-                        const Geom &g, int nblk, hipStream_t s) {
+  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float2 *out,
+                        const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
     constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                       rec, grid, out, g);
+                       zr, grid, out, g, dpoly);
     return hipGetLastError();
   }
 };
@@ -1617,15 +1670,25 @@ int spread_block_threads(int w, int method) {
   return w >= 8 ? 512 : 256;
 }
 
+hipError_t launch_point_z(const PointArgs &pa, const float2 *values, const Geom &g, ZRec zr,
+                          hipStream_t s) {
+  if (pa.num_points <= 0) return hipSuccess;
+  int64_t blocks = (pa.num_points + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(point_z_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pa, values, g, zr);
+  return hipGetLastError();
+}
+
 hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
-                           const uint8_t *xo, float2 *grid, const Geom &g, const KbPoly &poly,
-                           int nblk, hipStream_t s) {
-  if (method == kSpreadMfma) return dispatch_w<Spread3Mfma>(w, pa, rec, xo, grid, g, poly, nblk, s);
+                           const uint8_t *xo, ZRec zr, float2 *grid, const Geom &g,
+                           const KbPoly &poly, const KbPoly *dpoly, int nblk, hipStream_t s) {
+  if (method == kSpreadMfma) return dispatch_w<Spread3Mfma>(w, pa, zr, grid, g, dpoly, nblk, s);
   return dispatch_w<Spread3>(w, pa, rec, xo, grid, g, poly, nblk, s);
 }
 hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_probs, int nprob,
                                    SpreadProblem *dev_table, uint32_t *dev_prefix,
-                                   const KbPoly &poly, int nblk, hipStream_t s) {
+                                   const KbPoly &poly, const KbPoly *dpoly, int nblk,
+                                   hipStream_t s) {
   for (int first = 0; first < nprob; first += kBatchChunk) {
     BatchChunk chunk{};
     chunk.first = first;
@@ -1638,15 +1701,16 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
   if (e != hipSuccess) return e;
   if (method == kSpreadMfma)
     return dispatch_w<Spread3MfmaBatched>(w, (const SpreadProblem *)dev_table,
-                                          (const uint32_t *)dev_prefix, nprob, poly, nblk, s);
+                                          (const uint32_t *)dev_prefix, nprob, dpoly, nblk, s);
   return dispatch_w<Spread3Batched>(w, (const SpreadProblem *)dev_table, (const uint32_t *)dev_prefix,
                                     nprob, poly, nblk, s);
 }
 
-hipError_t launch_interp3d(int w, int method, const PointArgs &pa, const float *rec,
-                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
+                           float2 *out, const Geom &g, const KbPoly &poly, const KbPoly *dpoly,
                            int nblk, hipStream_t s) {
-  if (method == kSpreadMfma) return dispatch_w<Interp3Mfma>(w, pa, rec, grid, out, g, nblk, s);
+  if (method == kSpreadMfma)
+    return dispatch_w<Interp3Mfma>(w, pa, zr, grid, out, g, dpoly, nblk, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
