@@ -101,7 +101,6 @@ struct PointArgs {
   const uint32_t *perm;      // sorted order -> original point index
   const int4 *subs;
   const uint32_t *nsub;      // device count of subproblems
-  float *sx, *sy, *sz;       // coordinates in sorted order (filled by the prep kernels)
 };
 
 // Compact per-point records in sorted order for the MFMA kernels (SoA):
@@ -149,8 +148,6 @@ hipError_t launch_point_z(const PointArgs &pa, const float2 *values, const Geom 
                           hipStream_t s);
 hipError_t launch_point_prep(int w, const PointArgs &pa, const float2 *values, const Geom &g,
                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
-// Coordinates in sorted order for the MFMA interpolation / gradient kernels.
-hipError_t launch_gather_sorted(const PointArgs &pa, hipStream_t s);
 hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
                            const uint8_t *xo, ZRec zr, float2 *grid, const Geom &g,
                            const KbPoly &poly, const KbPoly *dpoly, int num_blocks, hipStream_t s);
@@ -161,9 +158,9 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float2 *out, const Geom &g, const KbPoly &poly, const KbPoly *dpoly,
                            int num_blocks, hipStream_t s);
-hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
+hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
-                         const KbPoly &poly, int num_blocks, hipStream_t s);
+                         const KbPoly &poly, const KbPoly *dpoly, int num_blocks, hipStream_t s);
 
 hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t num_points,
                            const float2 *values, float2 *grid, const Geom &g,

@@ -182,7 +182,7 @@ struct StageTimer {
 
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
-         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, srt = 0, grid = 0, fftbuf = 0,
+         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, grid = 0, fftbuf = 0,
          total = 0;
   int nchunks = 0;
 };
@@ -272,7 +272,6 @@ Layout layout(const anufft_plan *p, int64_t M) {
     else
       l.rec = take(sizeof(float) * (size_t)point_rec_floats(p->w) * Mp);
     l.xo = take(Mp);
-    l.srt = take(3 * 4 * Mp);
     l.grid = take(8 * (size_t)p->cells());
     l.fftbuf = take(8 * (size_t)(p->m[0] * p->m[1] * p->n[2]));
   } else {
@@ -689,12 +688,6 @@ struct Exec {
     pa.perm = sb.idx[b.passes & 1];
     pa.subs = sb.subs;
     pa.nsub = sb.sub_ofs + b.nbins;
-    if (lay.srt) {
-      const size_t Mp = (size_t)std::max<int64_t>(M, 1);
-      pa.sx = (float *)(ws + lay.srt);
-      pa.sy = pa.sx + Mp;
-      pa.sz = pa.sy + Mp;
-    }
     return pa;
   }
   int blocks(const anufft_plan::Binning &b, int64_t M, int per_cu) const {
@@ -991,14 +984,14 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
     e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, b.g, 3, ex.sort_buffers(b), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "binning");
     if (gm) {
-      e = ex.timed(ANUFFT_STAGE_PREP, [&] { return launch_gather_sorted(pa, ex.s); });
-      if (e != hipSuccess) return hip_fail(e, "sorted coordinates");
+      e = ex.prep(b, pa, nullptr);
+      if (e != hipSuccess) return hip_fail(e, "point records");
     }
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
-      return launch_grad3d(p->w, p->kernel_method, pa, ex.grid(), (const float2 *)u, grad[0],
-                           grad[1], grad[2], b.g, p->poly,
+      return launch_grad3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(), (const float2 *)u,
+                           grad[0], grad[1], grad[2], b.g, p->poly, p->d_poly,
                            ex.blocks(b, M, gm ? 1 : 4), ex.s);
     });
   } else {

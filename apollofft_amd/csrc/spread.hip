@@ -192,9 +192,10 @@ template <int W>
 struct TapPoly {
   float c[kPolyCoefs];
   int i;
-  __device__ __forceinline__ TapPoly(const KbPoly *__restrict__ P, int tap) : i(tap) {
+  // coefficient table coef[d][tap]: KbPoly::c (values) or KbPoly::dc (derivatives)
+  __device__ __forceinline__ TapPoly(const float (*__restrict__ coef)[kMaxTaps], int tap) : i(tap) {
 #pragma unroll
-    for (int d = 0; d < kPolyCoefs; ++d) c[d] = P->c[d][tap];
+    for (int d = 0; d < kPolyCoefs; ++d) c[d] = coef[d][tap];
   }
   __device__ __forceinline__ float operator()(float z) const {
     float v = c[kPolyCoefs - 1];
@@ -205,18 +206,6 @@ struct TapPoly {
     return v;
   }
 };
-
-// Coordinates in sorted order (one gather through the permutation), so the
-// MFMA interpolation and gradient kernels stage points with coalesced loads.
-__global__ __launch_bounds__(256) void gather_sorted_kernel(PointArgs pa) {
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < pa.num_points; s += st) {
-    const int64_t j = (int64_t)pa.perm[s] * pa.stride;
-    pa.sx[s] = pa.x[j];
-    pa.sy[s] = pa.y[j];
-    pa.sz[s] = pa.z[j];
-  }
-}
 
 // LDS staging of the y and z weights of one point, zero-padded to the 32
 // column positions of the padded tile, so a lane reads its weight at its own
@@ -458,20 +447,6 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
-// Tap weights of one axis written into a zero-filled LDS row or column:
-// dst[(l + i) * step] = phi_i for i in [0, 2w), from the coefficients pc.
-template <int W>
-__device__ __forceinline__ void put_taps(float *dst, int step, int l, float z, const float *pc) {
-#pragma unroll 1
-  for (int i = 0; i < 2 * W; ++i) {
-    float v = pc[(kPolyCoefs - 1) * kMaxTaps + i];
-#pragma unroll
-    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, pc[d * kMaxTaps + i]);
-    if (i == 0 && z <= -1.0f) v = 0.0f;
-    if (i == 2 * W - 1 && z >= 1.0f) v = 0.0f;
-    dst[(l + i) * step] = v;
-  }
-}
 
 template <int W>
 __device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *grid, const Geom &g,
@@ -554,7 +529,7 @@ __device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *g
     // pass 1: tap weights; each wave evaluates whole taps (coefficients in
     // scalar registers, loaded once per tap) for all points and axes
     for (int i0 = wave; i0 < T; i0 += NW) {
-      const TapPoly<W> tp(dpoly, __builtin_amdgcn_readfirstlane(i0));
+      const TapPoly<W> tp(dpoly->c, __builtin_amdgcn_readfirstlane(i0));
       const int i = tp.i;
       for (int p = lane; p < np16; p += 64) {
         const int off = OFS[p];
@@ -756,14 +731,6 @@ struct ILds {
 };
 static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
 
-// Weights of one axis into a zero-filled staged row: row[l + i] = phi_i.
-template <int W>
-__device__ __forceinline__ void stage_axis(float *row, int l, double t, int cell, const float *pc) {
-#pragma unroll
-  for (int i = 0; i < 32; i += 4) *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-  const float z = (float)(1.0 - 2.0 * (t - (double)cell));
-  put_taps<W>(row, 1, l, z, pc);
-}
 
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
@@ -872,7 +839,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
       // pass 1: tap weights; each wave evaluates whole taps (coefficients
       // in scalar registers) for all points and axes
       for (int i0 = wave; i0 < 2 * W; i0 += NW) {
-        const TapPoly<W> tp(dpoly, __builtin_amdgcn_readfirstlane(i0));
+        const TapPoly<W> tp(dpoly->c, __builtin_amdgcn_readfirstlane(i0));
         const int i = tp.i;
         for (int p = lane; p < np32; p += 64) {
           const int off = OFS[p];
@@ -970,31 +937,17 @@ struct GLds {
   static constexpr int ROWS = 0;                                   // float [6][kGNP][kIRow]
   static constexpr int RED = ROWS + 6 * kGNP * kIRow * 4;          // float [NW][6][kGNP]
   static constexpr int JIDX = RED + NW * 6 * kGNP * 4;             // int [kGNP]
-  static constexpr int PC = JIDX + kGNP * 4;                       // float [2][kPolyCoefs][kMaxTaps]
-  static constexpr int TMX = PC + 2 * kPolyCoefs * kMaxTaps * 4;   // float [16]
+  static constexpr int ZS = JIDX + kGNP * 4;                       // float [3][kGNP]
+  static constexpr int OFS = ZS + 3 * kGNP * 4;                    // int [kGNP]
+  static constexpr int TMX = OFS + kGNP * 4;                       // float [16]
   static constexpr int SIZE = TMX + 16 * 4;
 };
 
-// Values and derivatives of one axis into zero-filled rows.
-template <int W>
-__device__ __forceinline__ void stage_axis_deriv(float *row, float *drow, int l, double t, int cell,
-                                                 const float *pc, const float *pdc) {
-#pragma unroll
-  for (int i = 0; i < 32; i += 4) {
-    *(float4 *)(row + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-    *(float4 *)(drow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-  const float z = (float)(1.0 - 2.0 * (t - (double)cell));
-  put_taps<W>(row, 1, l, z, pc);
-  put_taps<W>(drow, 1, l, z, pdc);
-}
 
 template <int W>
-__global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(PointArgs pa,
-                                                                      const float2 *grid,
-                                                                      const float2 *u, float *gx,
-                                                                      float *gy, float *gz, Geom g,
-                                                                      KbPoly poly) {
+__global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
+    PointArgs pa, ZRec zr, const float2 *grid, const float2 *u, float *gx, float *gy, float *gz,
+    Geom g, const KbPoly *__restrict__ dpoly) {
   using MG = MGrad<W>;
   constexpr int NT = MG::NT, XW = MG::XW, NW = MG::NW;
   using L = GLds<NW>;
@@ -1002,13 +955,9 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(PointArgs 
   float *ROWS = (float *)(shm + L::ROWS);  // WX, WY, WZ, DWX, DWY, DWZ
   float *RED = (float *)(shm + L::RED);
   int *JIDX = (int *)(shm + L::JIDX);
-  float *pc = (float *)(shm + L::PC);
-  float *pdc = pc + kPolyCoefs * kMaxTaps;
+  float *ZS = (float *)(shm + L::ZS);
+  int *OFS = (int *)(shm + L::OFS);
   float *tmx = (float *)(shm + L::TMX);
-  for (int i = threadIdx.x; i < kPolyCoefs * kMaxTaps; i += NT) {
-    pc[i] = poly.c[i / kMaxTaps][i % kMaxTaps];
-    pdc[i] = poly.dc[i / kMaxTaps][i % kMaxTaps];
-  }
   auto row = [&](int which, int p) { return ROWS + (which * kGNP + p) * kIRow; };
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1077,22 +1026,32 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(PointArgs 
       const int np32 = ng << 5;
       const int start = sub.y + c0;
       __syncthreads();
-      for (int t = threadIdx.x; t < 3 * np32; t += NT) {
-        const int ax = t / np32;
-        const int p = t - ax * np32;
-        float *r0 = row(ax, p), *r1 = row(3 + ax, p);
-        if (p < npts) {
-          const float *cp = ax == 0 ? pa.sx : (ax == 1 ? pa.sy : pa.sz);
-          int cell;
-          const double tt = fold_coord(cp[start + p], g.scale[ax], g.m[ax], cell);
-          const int l = ax == 0 ? cell - bx0 : cell - W + 1 - (ax == 1 ? oy : oz);
-          stage_axis_deriv<W>(r0, r1, l, tt, cell, pc, pdc);
-          if (ax == 0) JIDX[p] = (int)pa.perm[start + p];
-        } else {
+      // pass 0: zero the six staged rows; arguments, offsets, output indices
+      for (int p = threadIdx.x; p < np32; p += NT) {
 #pragma unroll
-          for (int i = 0; i < 32; i += 4) {
-            *(float4 *)(r0 + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-            *(float4 *)(r1 + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int r = 0; r < 6; ++r) {
+#pragma unroll
+          for (int i = 0; i < 32; i += 4) *(float4 *)(row(r, p) + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const bool ok = p < npts;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ZS[a * kGNP + p] = ok ? zr.z[a][start + p] : 0.f;
+        OFS[p] = ok ? zr.off[start + p] : 0;
+        if (ok) JIDX[p] = (int)pa.perm[start + p];
+      }
+      __syncthreads();
+      // pass 1: value and derivative taps, whole taps per wave
+      for (int i0 = wave; i0 < 2 * W; i0 += NW) {
+        const int i = __builtin_amdgcn_readfirstlane(i0);
+        const TapPoly<W> tv(dpoly->c, i), td(dpoly->dc, i);
+        for (int p = lane; p < np32; p += 64) {
+          const int off = OFS[p];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float z = ZS[a * kGNP + p];
+            const int l = ((off >> (8 * a)) & 255) + i;
+            row(a, p)[l] = tv(z);
+            row(3 + a, p)[l] = td(z);
           }
         }
       }
@@ -1564,16 +1523,16 @@ struct Interp3 {
 
 template <int W>
 struct Grad3Mfma {
-  static hipError_t run(const PointArgs &pa, const float2 *grid, const float2 *u, float *gx,
-                        float *gy, float *gz, const Geom &g, const KbPoly &poly, int nblk,
-                        hipStream_t s) {
+  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, const float2 *u,
+                        float *gx, float *gy, float *gz, const Geom &g, const KbPoly *dpoly,
+                        int nblk, hipStream_t s) {
     if constexpr (W < 8) {
       constexpr int lds = GLds<MGrad<W>::NW>::SIZE;
       hipError_t e = hipFuncSetAttribute((const void *)grad3d_mfma_kernel<W>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(grad3d_mfma_kernel<W>, dim3(nblk), dim3(MGrad<W>::NT), lds, s, pa, grid,
-                         u, gx, gy, gz, g, poly);
+      hipLaunchKernelGGL(grad3d_mfma_kernel<W>, dim3(nblk), dim3(MGrad<W>::NT), lds, s, pa, zr,
+                         grid, u, gx, gy, gz, g, dpoly);
       return hipGetLastError();
     } else {
       return hipErrorInvalidValue;  // w = 8 uses the LDS-tile kernel (grad_uses_mfma)
@@ -1646,13 +1605,6 @@ struct LdsSize {
 
 int point_rec_floats(int w) { return (2 + 6 * w + 1 + 3) / 4 * 4; }
 
-hipError_t launch_gather_sorted(const PointArgs &pa, hipStream_t s) {
-  if (pa.num_points <= 0) return hipSuccess;
-  int64_t blocks = (pa.num_points + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(gather_sorted_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pa);
-  return hipGetLastError();
-}
 
 hipError_t launch_point_prep(int w, const PointArgs &pa, const float2 *values, const Geom &g,
                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s) {
@@ -1713,11 +1665,11 @@ hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, cons
     return dispatch_w<Interp3Mfma>(w, pa, zr, grid, out, g, dpoly, nblk, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
-hipError_t launch_grad3d(int w, int method, const PointArgs &pa, const float2 *grid,
+hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
-                         const KbPoly &poly, int nblk, hipStream_t s) {
+                         const KbPoly &poly, const KbPoly *dpoly, int nblk, hipStream_t s) {
   if (grad_uses_mfma(w, method))
-    return dispatch_w<Grad3Mfma>(w, pa, grid, u, gx, gy, gz, g, poly, nblk, s);
+    return dispatch_w<Grad3Mfma>(w, pa, zr, grid, u, gx, gy, gz, g, dpoly, nblk, s);
   return dispatch_w<Grad3>(w, pa, grid, u, gx, gy, gz, g, poly, nblk, s);
 }
 hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t n,
