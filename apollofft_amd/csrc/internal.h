@@ -155,9 +155,10 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
                                    SpreadProblem *dev_table, uint32_t *dev_prefix,
                                    const KbPoly &poly, const KbPoly *dpoly, int num_blocks,
                                    hipStream_t s);
+// MFMA interpolation needs 2 nbins floats of scratch (bin and tile maxima).
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
-                           float2 *out, const Geom &g, const KbPoly &poly, const KbPoly *dpoly,
-                           int num_blocks, hipStream_t s);
+                           float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
+                           const KbPoly *dpoly, int num_blocks, hipStream_t s);
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
                          const KbPoly &poly, const KbPoly *dpoly, int num_blocks, hipStream_t s);

@@ -182,7 +182,7 @@ struct StageTimer {
 
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
-         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, grid = 0, fftbuf = 0,
+         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, tmax = 0, grid = 0, fftbuf = 0,
          total = 0;
   int nchunks = 0;
 };
@@ -267,8 +267,10 @@ Layout layout(const anufft_plan *p, int64_t M) {
     for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
       nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
     l.subs = take(16 * (size_t)nsub);
-    if (p->kernel_method == kSpreadMfma)
+    if (p->kernel_method == kSpreadMfma) {
       l.zrec = take(24 * Mp);  // z[3], off (4 B each), value (8 B) per point
+      l.tmax = take(2 * sizeof(float) * (size_t)(p->bi.nbins + 1));  // bin and tile maxima
+    }
     else
       l.rec = take(sizeof(float) * (size_t)point_rec_floats(p->w) * Mp);
     l.xo = take(Mp);
@@ -947,8 +949,8 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(), (float2 *)out,
-                             p->bi.g, p->poly, p->d_poly,
+      return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(),
+                             (float *)(ex.ws + lay.tmax), (float2 *)out, p->bi.g, p->poly, p->d_poly,
                              ex.blocks(p->bi, M, p->kernel_method == kSpreadMfma ? 2 : 4), ex.s);
     });
   } else {

@@ -37,6 +37,30 @@ constexpr int kSpreadStage = 256;  // points staged per round (spread)
 constexpr int kSpreadThreads = 1024;
 constexpr int kInterpThreads = 1024;
 
+// XCD-aware split of the subproblem list.  Workgroups are dealt to the 8
+// XCDs of the chip round-robin (block b runs on XCD b mod 8); each XCD takes
+// one contiguous eighth of the list (neighbouring tiles, whose atomics and
+// halo reads then meet in that XCD's L2) and its blocks stride through it.
+struct XcdRange {
+  uint32_t begin, end, step;
+  __device__ __forceinline__ XcdRange(uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    if (g < kXcd) {
+      begin = b;
+      end = n;
+      step = g;
+      return;
+    }
+    const uint32_t x = b % kXcd, j = b / kXcd;
+    const uint32_t per = (g - x + kXcd - 1) / kXcd;  // blocks on XCD x
+    const uint64_t lo = (uint64_t)n * x / kXcd, hi = (uint64_t)n * (x + 1) / kXcd;
+    begin = (uint32_t)lo + j;
+    end = (uint32_t)hi;
+    step = per;
+  }
+};
+
 // Periodic index; padded tiles may exceed the grid by more than one period
 // when the grid is not much larger than the kernel support.
 __device__ __forceinline__ int wrap(int v, int m) {
@@ -627,8 +651,8 @@ template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
     PointArgs pa, ZRec zr, float2 *grid, Geom g, const KbPoly *__restrict__ dpoly) {
   extern __shared__ __align__(16) char shm[];
-  const uint32_t nsub = *pa.nsub;
-  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x)
+  const XcdRange r(*pa.nsub);
+  for (uint32_t sidx = r.begin; sidx < r.end; sidx += r.step)
     spread_subproblem_mfma<W>(zr, grid, g, dpoly, pa.subs[sidx], shm);
 }
 
@@ -706,6 +730,69 @@ __device__ __forceinline__ void load_tile(const float2 *grid, const Geom &g, int
   }
 }
 
+// ---- tile maxima of the grid (scales of the MFMA interpolation tiles) -------
+//
+// bmax[b] = largest |re| or |im| over the cells of bin b (one block per bin);
+// tmax[b] = largest bmax over the bins that the padded tile of bin b
+// overlaps, i.e. the largest component of that tile.
+__global__ __launch_bounds__(256) void bin_max_kernel(const float2 *grid, Geom g, float *bmax) {
+  __shared__ float red[4];
+  const int bin = blockIdx.x;
+  const int bz = bin % g.nbin[2];
+  const int by = (bin / g.nbin[2]) % g.nbin[1];
+  const int bx = bin / (g.nbin[2] * g.nbin[1]);
+  const int x0 = bx * g.bin[0], y0 = by * g.bin[1], z0 = bz * g.bin[2];
+  const int nx = min(g.bin[0], g.m[0] - x0), ny = min(g.bin[1], g.m[1] - y0),
+            nz = min(g.bin[2], g.m[2] - z0);
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < nx * ny * nz; i += blockDim.x) {
+    const int z = i % nz, y = (i / nz) % ny, x = i / (nz * ny);
+    const float2 v = grid[((int64_t)(x0 + x) * g.m[1] + y0 + y) * g.m[2] + z0 + z];
+    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) bmax[bin] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+// Distinct bins (in order, periodic) of the n cells starting at unwrapped c0.
+__device__ __forceinline__ int cell_bins(int c0, int n, int m, int bin, int *out) {
+  int k = 0, last = -1;
+  for (int i = 0; i < n && k < 40; ++i) {
+    int c = c0 + i;
+    while (c < 0) c += m;
+    while (c >= m) c -= m;
+    const int b = c / bin;
+    if (b != last) {
+      bool seen = false;
+      for (int t = 0; t < k; ++t) seen |= out[t] == b;
+      if (!seen) out[k++] = b;
+      last = b;
+    }
+  }
+  return k;
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void tile_max_kernel(const float *bmax, Geom g, float *tmax) {
+  const int bin = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
+  if (bin >= nbins) return;
+  const int bz = bin % g.nbin[2];
+  const int by = (bin / g.nbin[2]) % g.nbin[1];
+  const int bx = bin / (g.nbin[2] * g.nbin[1]);
+  int lx[40], ly[40], lz[40];
+  const int nx = cell_bins(bx * g.bin[0] - W + 1, MSpread<W>::PX, g.m[0], g.bin[0], lx);
+  const int ny = cell_bins(by * g.bin[1] - W, g.pad[1], g.m[1], g.bin[1], ly);
+  const int nz = cell_bins(bz * g.bin[2] - W, g.pad[2], g.m[2], g.bin[2], lz);
+  float mx = 0.f;
+  for (int a = 0; a < nx; ++a)
+    for (int b = 0; b < ny; ++b)
+      for (int c = 0; c < nz; ++c) mx = fmaxf(mx, bmax[(lx[a] * g.nbin[1] + ly[b]) * g.nbin[2] + lz[c]]);
+  tmax[bin] = mx;
+}
+
 // ---- MFMA interpolation -----------------------------------------------------
 //
 // Same tile geometry as MFMA spreading.  For x cell i the wave computes
@@ -734,7 +821,7 @@ static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
 
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
-    PointArgs pa, ZRec zr, const float2 *grid, float2 *out, Geom g,
+    PointArgs pa, ZRec zr, const float2 *grid, const float *tmax, float2 *out, Geom g,
     const KbPoly *__restrict__ dpoly) {
   using MS = MSpread<W>;
   constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW;
@@ -747,12 +834,11 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
   int *JIDX = (int *)(shm + L::JIDX);
   float *ZS = (float *)(shm + L::ZS);
   int *OFS = (int *)(shm + L::OFS);
-  float *tmx = (float *)(shm + L::TMX);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-  const uint32_t nsub = *pa.nsub;
-  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x) {
+  const XcdRange xr(*pa.nsub);
+  for (uint32_t sidx = xr.begin; sidx < xr.end; sidx += xr.step) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
@@ -764,36 +850,14 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
     // A operands: lane (l32, h) holds G[x][y = l32][z] for z = 8h + j (step
     // 0) and 16 + 8h + j (step 1) of its XW x cells; columns outside the
     // padded tile are zero.
-    // The tile is read twice (the second time from L2): once for its
-    // largest component, once to scale and split, so the raw values need no
-    // registers of their own.
-    float mx = 0.f;
-    const bool yok = l32 < g.pad[1];
-    const int gy = wrap(oy + l32, g.m[1]);
-#pragma unroll
-    for (int q = 0; q < XW; ++q) {
-      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
-      const float2 *rowp = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int z = (j < 8 ? 0 : 8) + 8 * h + j;
-        if (yok && z < g.pad[2]) {
-          const float2 v = rowp[wrap(oz + z, g.m[2])];
-          mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-        }
-      }
-    }
-    mx = wave_max(mx);
-    __syncthreads();  // previous subproblem's readers of tmx / RED are done
-    if (lane == 0) tmx[wave] = mx;
-    __syncthreads();
-    mx = tmx[0];
-#pragma unroll
-    for (int i = 1; i < NW; ++i) mx = fmaxf(mx, tmx[i]);
+    // tile scale from the precomputed tile maximum (tile_max_kernel)
+    const float mx = tmax[bin];
     int e = 0;
     if (mx > 0.f) (void)frexpf(mx, &e);
     e = max(-120, min(120, e));
     const float s_in = ldexpf(1.0f, -e) * kOpScale, s_out = ldexpf(1.0f, e) * kProdUnscale;
+    const bool yok = l32 < g.pad[1];
+    const int gy = wrap(oy + l32, g.m[1]);
     half8 arh[XW][2], arl[XW][2], aih[XW][2], ail[XW][2];
 #pragma unroll
     for (int q = 0; q < XW; ++q) {
@@ -801,14 +865,30 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
       const float2 *rowp = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2];
 #pragma unroll
       for (int st = 0; st < 2; ++st) {
+        // 8 consecutive z columns: 16-byte loads unless the run wraps or
+        // leaves the padded tile
+        const int zc = 16 * st + 8 * h;
+        const int z0 = oz + zc;
+        float2 v[8];
+        const int64_t e0 = ((int64_t)gx * g.m[1] + gy) * g.m[2] + z0;  // element index
+        if (yok && zc + 8 <= g.pad[2] && z0 >= 0 && z0 + 8 <= g.m[2] && (e0 & 1) == 0) {
+          const float4 *src = (const float4 *)(rowp + z0);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float4 t = src[j];
+            v[2 * j] = make_float2(t.x, t.y);
+            v[2 * j + 1] = make_float2(t.z, t.w);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            v[j] = (yok && zc + j < g.pad[2]) ? rowp[wrap(z0 + j, g.m[2])] : make_float2(0.f, 0.f);
+        }
         float tr[8], ti[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const int z = 16 * st + 8 * h + j;
-          float2 v = make_float2(0.f, 0.f);
-          if (yok && z < g.pad[2]) v = rowp[wrap(oz + z, g.m[2])];
-          tr[j] = v.x * s_in;
-          ti[j] = v.y * s_in;
+          tr[j] = v[j].x * s_in;
+          ti[j] = v[j].y * s_in;
         }
         split8(tr, arh[q][st], arl[q][st]);
         split8(ti, aih[q][st], ail[q][st]);
@@ -962,8 +1042,8 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-  const uint32_t nsub = *pa.nsub;
-  for (uint32_t sidx = blockIdx.x; sidx < nsub; sidx += gridDim.x) {
+  const XcdRange xr(*pa.nsub);
+  for (uint32_t sidx = xr.begin; sidx < xr.end; sidx += xr.step) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
@@ -1495,14 +1575,19 @@ struct Spread3Batched {
 
 template <int W>
 struct Interp3Mfma {
-  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float2 *out,
-                        const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
+  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
+                        float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
+    const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
+    float *tmax = bmax + nbins;
+    hipLaunchKernelGGL(bin_max_kernel, dim3(nbins), dim3(256), 0, s, grid, g, bmax);
+    hipLaunchKernelGGL(tile_max_kernel<W>, dim3((nbins + 63) / 64), dim3(64), 0, s,
+                       (const float *)bmax, g, tmax);
     constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                       zr, grid, out, g, dpoly);
+                       zr, grid, (const float *)tmax, out, g, dpoly);
     return hipGetLastError();
   }
 };
@@ -1659,10 +1744,10 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
 }
 
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
-                           float2 *out, const Geom &g, const KbPoly &poly, const KbPoly *dpoly,
-                           int nblk, hipStream_t s) {
+                           float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
+                           const KbPoly *dpoly, int nblk, hipStream_t s) {
   if (method == kSpreadMfma)
-    return dispatch_w<Interp3Mfma>(w, pa, zr, grid, out, g, dpoly, nblk, s);
+    return dispatch_w<Interp3Mfma>(w, pa, zr, grid, scratch, out, g, dpoly, nblk, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
