@@ -756,41 +756,31 @@ __global__ __launch_bounds__(256) void bin_max_kernel(const float2 *grid, Geom g
   if (threadIdx.x == 0) bmax[bin] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
-// Distinct bins (in order, periodic) of the n cells starting at unwrapped c0.
-__device__ __forceinline__ int cell_bins(int c0, int n, int m, int bin, int *out) {
-  int k = 0, last = -1;
-  for (int i = 0; i < n && k < 40; ++i) {
-    int c = c0 + i;
-    while (c < 0) c += m;
-    while (c >= m) c -= m;
-    const int b = c / bin;
-    if (b != last) {
-      bool seen = false;
-      for (int t = 0; t < k; ++t) seen |= out[t] == b;
-      if (!seen) out[k++] = b;
-      last = b;
-    }
-  }
-  return k;
-}
-
-template <int W>
-__global__ __launch_bounds__(64) void tile_max_kernel(const float *bmax, Geom g, float *tmax) {
+// One axis of the tile maxima, separably: dst[b] = max of src over the bins
+// (along `axis`) that the tile of bin b covers along that axis (cells
+// [b * bin + org, + ext), periodic).  Three launches (x, y, z) turn bin
+// maxima into tile maxima.
+__global__ __launch_bounds__(64) void tile_max_axis_kernel(const float *src, Geom g, int axis,
+                                                           int org, int ext, float *dst) {
   const int bin = blockIdx.x * blockDim.x + threadIdx.x;
   const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
   if (bin >= nbins) return;
-  const int bz = bin % g.nbin[2];
-  const int by = (bin / g.nbin[2]) % g.nbin[1];
-  const int bx = bin / (g.nbin[2] * g.nbin[1]);
-  int lx[40], ly[40], lz[40];
-  const int nx = cell_bins(bx * g.bin[0] - W + 1, MSpread<W>::PX, g.m[0], g.bin[0], lx);
-  const int ny = cell_bins(by * g.bin[1] - W, g.pad[1], g.m[1], g.bin[1], ly);
-  const int nz = cell_bins(bz * g.bin[2] - W, g.pad[2], g.m[2], g.bin[2], lz);
+  int b3[3];
+  b3[2] = bin % g.nbin[2];
+  b3[1] = (bin / g.nbin[2]) % g.nbin[1];
+  b3[0] = bin / (g.nbin[2] * g.nbin[1]);
+  const int m = g.m[axis], bw = g.bin[axis];
+  int c = b3[axis] * bw + org;
+  while (c < 0) c += m;
+  while (c >= m) c -= m;
   float mx = 0.f;
-  for (int a = 0; a < nx; ++a)
-    for (int b = 0; b < ny; ++b)
-      for (int c = 0; c < nz; ++c) mx = fmaxf(mx, bmax[(lx[a] * g.nbin[1] + ly[b]) * g.nbin[2] + lz[c]]);
-  tmax[bin] = mx;
+  for (int i = 0; i < ext; ++i) {
+    int n3[3] = {b3[0], b3[1], b3[2]};
+    n3[axis] = c / bw;
+    mx = fmaxf(mx, src[(n3[0] * g.nbin[1] + n3[1]) * g.nbin[2] + n3[2]]);
+    if (++c == m) c = 0;
+  }
+  dst[bin] = mx;
 }
 
 // ---- MFMA interpolation -----------------------------------------------------
@@ -1577,11 +1567,18 @@ template <int W>
 struct Interp3Mfma {
   static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
                         float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
+    // bin maxima -> (x) -> (y) -> (z): tile maxima in tmax, ping-ponging
+    // between the two halves of the scratch
     const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
     float *tmax = bmax + nbins;
+    const unsigned tb = (unsigned)((nbins + 63) / 64);
     hipLaunchKernelGGL(bin_max_kernel, dim3(nbins), dim3(256), 0, s, grid, g, bmax);
-    hipLaunchKernelGGL(tile_max_kernel<W>, dim3((nbins + 63) / 64), dim3(64), 0, s,
-                       (const float *)bmax, g, tmax);
+    hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)bmax, g, 0,
+                       1 - W, MSpread<W>::PX, tmax);
+    hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)tmax, g, 1,
+                       -W, g.pad[1], bmax);
+    hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)bmax, g, 2,
+                       -W, g.pad[2], tmax);
     constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
