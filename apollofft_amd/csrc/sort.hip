@@ -11,7 +11,8 @@
 //   keys   : key[j] = tile of point j, per-chunk histogram of digit 0
 //   per digit pass: scan(counts) -> stable scatter (-> histogram of next digit)
 //   bounds : bin_start[b] = lower_bound(sorted keys, b)
-//   subs   : subproblems of at most max_sub points per tile
+//   subs   : subproblems of at most max_sub points per tile, tiles in pencil
+//            order (x fastest)
 #include "internal.h"
 
 namespace anufft {
@@ -217,24 +218,34 @@ __global__ void bounds_kernel(const uint32_t *keys, int64_t n, uint32_t *bin_sta
   bin_start[b] = (uint32_t)lo;
 }
 
-__global__ void subcount_kernel(const uint32_t *bin_start, int nbins, int max_sub,
+// The subproblem list runs over the tiles in pencil order -- x fastest,
+// then z, then y -- so that consecutive entries are neighbouring tiles
+// along x; q is the position of a tile in that order.
+__device__ __forceinline__ int pencil_bin(int q, int3 nb) {
+  const int bx = q % nb.x, pz = q / nb.x;
+  return (bx * nb.y + pz / nb.z) * nb.z + pz % nb.z;
+}
+
+__global__ void subcount_kernel(const uint32_t *bin_start, int nbins, int3 nb, int max_sub,
                                 uint32_t *sub_ofs) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b > nbins) return;
-  if (b == nbins) {
-    sub_ofs[b] = 0;
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > nbins) return;
+  if (q == nbins) {
+    sub_ofs[q] = 0;
     return;
   }
+  const int b = pencil_bin(q, nb);
   uint32_t c = bin_start[b + 1] - bin_start[b];
-  sub_ofs[b] = (c + (uint32_t)max_sub - 1) / (uint32_t)max_sub;
+  sub_ofs[q] = (c + (uint32_t)max_sub - 1) / (uint32_t)max_sub;
 }
 
 __global__ void subwrite_kernel(const uint32_t *bin_start, const uint32_t *sub_ofs, int nbins,
-                                int max_sub, int4 *subs) {
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
+                                int3 nb, int max_sub, int4 *subs) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nbins) return;
+  const int b = pencil_bin(q, nb);
   uint32_t start = bin_start[b], c = bin_start[b + 1] - start;
-  uint32_t o = sub_ofs[b];
+  uint32_t o = sub_ofs[q];
   for (uint32_t s = 0; s * (uint32_t)max_sub < c; ++s) {
     uint32_t cnt = c - s * (uint32_t)max_sub;
     if (cnt > (uint32_t)max_sub) cnt = (uint32_t)max_sub;
@@ -264,14 +275,15 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
   }
   const uint32_t *sorted = sb.keys[sb.passes & 1];
   int nb1 = sb.nbins + 1;
+  const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
   hipLaunchKernelGGL(bounds_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sorted, n,
                      sb.bin_start, sb.nbins);
   hipLaunchKernelGGL(subcount_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sb.bin_start,
-                     sb.nbins, sb.max_sub, sb.sub_ofs);
+                     sb.nbins, nb, sb.max_sub, sb.sub_ofs);
   hipError_t e = scan_exclusive(sb.sub_ofs, nb1, sb.scan_tmp, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(subwrite_kernel, dim3((sb.nbins + 255) / 256), dim3(256), 0, s,
-                     sb.bin_start, sb.sub_ofs, sb.nbins, sb.max_sub, sb.subs);
+                     sb.bin_start, sb.sub_ofs, sb.nbins, nb, sb.max_sub, sb.subs);
   return hipGetLastError();
 }
 

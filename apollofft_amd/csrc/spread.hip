@@ -471,11 +471,66 @@ __device__ __forceinline__ float wave_max(float v) {
   return v;
 }
 
+// Maximum of non-negative values over the 64 lanes (the DPP steps of
+// wave_sum; lanes a step does not write read 0, which is neutral here).
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  v = fmaxf(v, dpp<0xB1>(v));
+  v = fmaxf(v, dpp<0x4E>(v));
+  v = fmaxf(v, dpp<0x141>(v));
+  v = fmaxf(v, dpp<0x140>(v));
+  v = fmaxf(v, dpp<0x142, 0xa>(v));
+  v = fmaxf(v, dpp<0x143, 0xc>(v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+
+// Tile of a spreading subproblem.  The accumulators of a workgroup form a
+// ring of PX x columns: column x lives in slot x mod PX, so a workgroup that
+// moves on to the next tile along x keeps the columns both tiles cover and
+// flushes only those it leaves behind.
+struct MTile {
+  int pencil;       // y-z tile row (by * nbin_z + bz)
+  int bx;           // x tile index
+  int xlo, oy, oz;  // grid cell of tile column 0, row 0, layer 0
+};
 
 template <int W>
-__device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *grid, const Geom &g,
-                                                       const KbPoly *__restrict__ dpoly, int4 sub,
-                                                       char *sh) {
+__device__ __forceinline__ MTile mfma_tile(const Geom &g, int bin) {
+  const int bz = bin % g.nbin[2];
+  const int by = (bin / g.nbin[2]) % g.nbin[1];
+  const int bx = bin / (g.nbin[2] * g.nbin[1]);
+  return MTile{by * g.nbin[2] + bz, bx, bx * g.bin[0] - W + 1, by * g.bin[1] - W,
+               bz * g.bin[2] - W};
+}
+
+// Power-of-two scale of a subproblem's strengths: e with max |c| 2^-e in
+// [1/2, 1), clamped to [-120, 120] so that both scales stay finite and normal.
+template <int NW>
+__device__ __forceinline__ int mfma_strength_exp(const ZRec &zr, int4 sub, float *red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float mx = 0.f;
+  for (int p = threadIdx.x; p < sub.z; p += 64 * NW) {
+    const float2 c = zr.c[sub.y + p];
+    mx = fmaxf(mx, fmaxf(fabsf(c.x), fabsf(c.y)));
+  }
+  mx = wave_max_nonneg(mx);
+  __syncthreads();  // earlier readers of red[] are done
+  if (lane == 0) red[wave] = mx;
+  __syncthreads();
+  mx = red[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) mx = fmaxf(mx, red[i]);
+  int e = 0;
+  if (mx > 0.f) (void)frexpf(mx, &e);
+  return max(-120, min(120, e));
+}
+
+// Add the points of one subproblem, strengths scaled by s_in, to the ring
+// accumulators; base = tile column 0's ring slot.
+template <int W>
+__device__ __forceinline__ void mfma_accumulate(f32x16 (&acc)[MSpread<W>::XW][2], const ZRec &zr,
+                                                const KbPoly *__restrict__ dpoly, int4 sub,
+                                                int base, float s_in, char *sh) {
   using MS = MSpread<W>;
   constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW, T = 2 * W;
   using L = MLds<PX>;
@@ -486,45 +541,9 @@ __device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *g
   float *WZ = (float *)(sh + L::WZ);
   float *ZS = (float *)(sh + L::ZS);
   int *OFS = (int *)(sh + L::OFS);
-  float *red = (float *)(sh + L::RED);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-
-  const int bin = sub.x;
-  const int bz = bin % g.nbin[2];
-  const int by = (bin / g.nbin[2]) % g.nbin[1];
-  const int bx = bin / (g.nbin[2] * g.nbin[1]);
-  const int bx0 = bx * g.bin[0];
-  const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
-
-  // power-of-two scale of the strengths: max |c| * 2^-e in [1/2, 1)
-  float mx = 0.f;
-  for (int p = threadIdx.x; p < sub.z; p += NT) {
-    const float2 c = zr.c[sub.y + p];
-    mx = fmaxf(mx, fmaxf(fabsf(c.x), fabsf(c.y)));
-  }
-  mx = wave_max(mx);
-  __syncthreads();  // previous subproblem's readers of red[] are done
-  if (lane == 0) red[wave] = mx;
-  __syncthreads();
-  mx = red[0];
-#pragma unroll
-  for (int i = 1; i < MS::NW; ++i) mx = fmaxf(mx, red[i]);
-  int e = 0;
-  if (mx > 0.f) (void)frexpf(mx, &e);
-  e = max(-120, min(120, e));  // both scales stay finite and normal
-  const float s_in = ldexpf(1.0f, -e), s_out = ldexpf(1.0f, e) * kProdUnscale;
-
-  f32x16 acc[XW][2];
-#pragma unroll
-  for (int q = 0; q < XW; ++q) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      acc[q][0][r] = 0.f;
-      acc[q][1][r] = 0.f;
-    }
-  }
 
   for (int c0 = 0; c0 < sub.z; c0 += kMNP) {
     const int npts = min(kMNP, sub.z - c0);
@@ -551,13 +570,14 @@ __device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *g
     }
     __syncthreads();
     // pass 1: tap weights; each wave evaluates whole taps (coefficients in
-    // scalar registers, loaded once per tap) for all points and axes
+    // scalar registers, loaded once per tap) for all points and axes.  The
+    // x weights go to ring slot (tile column + base) mod PX.
     for (int i0 = wave; i0 < T; i0 += NW) {
       const TapPoly<W> tp(dpoly->c, __builtin_amdgcn_readfirstlane(i0));
       const int i = tp.i;
       for (int p = lane; p < np16; p += 64) {
         const int off = OFS[p];
-        WX[((off & 255) + i) * kMWY + p] = tp(ZS[p]) * kOpScale;
+        WX[(((off & 255) + i + base) & (PX - 1)) * kMWY + p] = tp(ZS[p]) * kOpScale;
         WY[(((off >> 8) & 255) + i) * kMWY + p] = tp(ZS[kMNP + p]);
         WZ[p * kMWZ + ((off >> 16) & 255) + i] = tp(ZS[2 * kMNP + p]);
       }
@@ -584,7 +604,8 @@ __device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *g
     }
     __syncthreads();
     // MFMA: 16 points per step; lane (l32, h) holds A row y = l32 and
-    // B column z = l32 for the points kb .. kb + 7
+    // B column z = l32 for the points kb .. kb + 7; wave w owns ring
+    // slots w XW .. w XW + XW - 1
     for (int gi = 0; gi < ng; ++gi) {
       const int kb = (gi << 4) + (h << 3);
       const half8 brh = *(const half8 *)(BT + (0 * 32 + l32) * kMBT + kb);
@@ -624,45 +645,159 @@ __device__ __forceinline__ void spread_subproblem_mfma(const ZRec &zr, float2 *g
       }
     }
   }
+}
 
-  // flush: accumulator r of lane (l32, h) is tile cell (y, z) with
-  // y = 8 (r / 4) + 4 h + r % 4, z = l32; x cell ix <-> x = bx0 - W + 1 + ix
-  if (l32 < g.pad[2]) {
-    const int gz = wrap(oz + l32, g.m[2]);
+// Add the ring columns x in [t.xlo, t.xlo + ncol) to the grid (scaled by
+// s_out) and zero them.  Accumulator r of lane (l32, h) is tile cell (y, z)
+// with y = 8 (r / 4) + 4 h + r % 4, z = l32.
+template <int W>
+__device__ __forceinline__ void mfma_flush(f32x16 (&acc)[MSpread<W>::XW][2], float2 *grid,
+                                           const Geom &g, const MTile &t, int ncol, float s_out) {
+  using MS = MSpread<W>;
+  constexpr int XW = MS::XW, PX = MS::PX;
+  int lane = threadIdx.x & 63;
+  // an opaque copy: keeps the per-row index arithmetic below from being
+  // hoisted out of the subproblem loop, where it would stay live in
+  // registers for the whole kernel
+  asm volatile("" : "+v"(lane));
+  const int wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  int m0 = __builtin_amdgcn_readfirstlane(g.m[0]);
+  int m1 = __builtin_amdgcn_readfirstlane(g.m[1]);
+  int m2 = __builtin_amdgcn_readfirstlane(g.m[2]);
+  asm volatile("" : "+s"(m0), "+s"(m1), "+s"(m2));  // no hoisted divisions by m
+  const int base = t.xlo & (PX - 1);
+  const bool zok = l32 < g.pad[2];
+  const int gz = wrap(t.oz + l32, m2);
 #pragma unroll
-    for (int q = 0; q < XW; ++q) {
-      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+  for (int q = 0; q < XW; ++q) {
+    const int ix = (wave * XW + q - base) & (PX - 1);  // tile column of the slot
+    if (ix >= ncol) continue;
+    if (zok) {
+      const int gx = wrap(t.xlo + ix, m0);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int y = ((r >> 2) << 3) + (h << 2) + (r & 3);
         const float vr = acc[q][0][r] * s_out, vi = acc[q][1][r] * s_out;
         if (y < g.pad[1] && (vr != 0.f || vi != 0.f)) {
-          const int gy = wrap(oy + y, g.m[1]);
-          float2 *dst = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2] + gz;
+          const int gy = wrap(t.oy + y, m1);
+          float2 *dst = grid + ((int64_t)gx * m1 + gy) * m2 + gz;
           unsafeAtomicAdd(&dst->x, vr);
           unsafeAtomicAdd(&dst->y, vi);
         }
       }
     }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      acc[q][0][r] = 0.f;
+      acc[q][1][r] = 0.f;
+    }
   }
 }
+
+// Ring window of one workgroup: the open tile, its problem and scale.
+struct MWindow {
+  int prob = -1;  // problem of the open tile, -1 = none
+  MTile t{};
+  int e = 0;
+};
+
+// Columns of the open window to flush before the window moves to tile n of
+// problem prob at strength exponent e (PX = all): a tile further along the
+// same pencil keeps the columns both cover, unless the scales are so far
+// apart that rescaling the kept sums could leave the normal FP32 range.
+template <int W>
+__device__ __forceinline__ int mfma_flush_cols(const MWindow &w, int prob, const MTile &n, int e,
+                                               const Geom &g) {
+  constexpr int PX = MSpread<W>::PX;
+  if (w.prob != prob || w.t.pencil != n.pencil || n.bx < w.t.bx || abs(e - w.e) > 32) return PX;
+  return min(PX, (n.bx - w.t.bx) * g.bin[0]);
+}
+
+template <int W>
+__device__ __forceinline__ void mfma_rescale(f32x16 (&acc)[MSpread<W>::XW][2], float f) {
+#pragma unroll
+  for (int q = 0; q < MSpread<W>::XW; ++q) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      acc[q][0][r] *= f;
+      acc[q][1][r] *= f;
+    }
+  }
+}
+
+// Contiguous run of the subproblem list for this workgroup: each XCD takes
+// one contiguous eighth (blocks are dealt to the XCDs round-robin, block b
+// on XCD b mod 8) and splits it evenly among its blocks, so a workgroup
+// walks consecutive tiles of a pencil.
+struct XcdChunk {
+  uint32_t begin, end;
+  __device__ __forceinline__ XcdChunk(uint32_t n) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t g = gridDim.x, b = blockIdx.x;
+    if (g < kXcd) {
+      begin = (uint32_t)((uint64_t)n * b / g);
+      end = (uint32_t)((uint64_t)n * (b + 1) / g);
+      return;
+    }
+    const uint32_t x = b % kXcd, j = b / kXcd;
+    const uint32_t per = (g - x + kXcd - 1) / kXcd;  // blocks on XCD x
+    const uint64_t lo = (uint64_t)n * x / kXcd, hi = (uint64_t)n * (x + 1) / kXcd;
+    begin = (uint32_t)(lo + (hi - lo) * j / per);
+    end = (uint32_t)(lo + (hi - lo) * (j + 1) / per);
+  }
+};
 
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
     PointArgs pa, ZRec zr, float2 *grid, Geom g, const KbPoly *__restrict__ dpoly) {
+  using MS = MSpread<W>;
   extern __shared__ __align__(16) char shm[];
-  const XcdRange r(*pa.nsub);
-  for (uint32_t sidx = r.begin; sidx < r.end; sidx += r.step)
-    spread_subproblem_mfma<W>(zr, grid, g, dpoly, pa.subs[sidx], shm);
+  float *red = (float *)(shm + MLds<MS::PX>::RED);
+  f32x16 acc[MS::XW][2];
+#pragma unroll
+  for (int q = 0; q < MS::XW; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
+  MWindow win;
+  const XcdChunk r(*pa.nsub);
+  for (uint32_t sidx = r.begin; sidx < r.end; ++sidx) {
+    const int4 sub = pa.subs[sidx];
+    const MTile t = mfma_tile<W>(g, sub.x);
+    const int e = mfma_strength_exp<MS::NW>(zr, sub, red);
+    if (win.prob >= 0) {
+      const int ncol = mfma_flush_cols<W>(win, 0, t, e, g);
+      mfma_flush<W>(acc, grid, g, win.t, ncol, ldexpf(1.0f, win.e) * kProdUnscale);
+      if (ncol < MS::PX && e != win.e) mfma_rescale<W>(acc, ldexpf(1.0f, win.e - e));
+    }
+    win.prob = 0;
+    win.t = t;
+    win.e = e;
+    mfma_accumulate<W>(acc, zr, dpoly, sub, t.xlo & (MS::PX - 1), ldexpf(1.0f, -e), shm);
+  }
+  if (win.prob >= 0)
+    mfma_flush<W>(acc, grid, g, win.t, MS::PX, ldexpf(1.0f, win.e) * kProdUnscale);
 }
 
+// Batched form: one persistent launch over the subproblems of all problems
+// (prefix[p] = first global subproblem index of problem p, prefix[nprob] =
+// total); the window moves on within a problem as above and is flushed
+// whole when the problem changes.
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kernel(
     const SpreadProblem *probs, const uint32_t *prefix, int nprob,
     const KbPoly *__restrict__ dpoly) {
+  using MS = MSpread<W>;
   extern __shared__ __align__(16) char shm[];
-  const uint32_t total = prefix[nprob];
-  for (uint32_t gidx = blockIdx.x; gidx < total; gidx += gridDim.x) {
+  float *red = (float *)(shm + MLds<MS::PX>::RED);
+  f32x16 acc[MS::XW][2];
+#pragma unroll
+  for (int q = 0; q < MS::XW; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
+  MWindow win;
+  const XcdChunk r(prefix[nprob]);
+  for (uint32_t gidx = r.begin; gidx < r.end; ++gidx) {
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -670,7 +805,23 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kerne
       else hi = mid - 1;
     }
     const SpreadProblem &pr = probs[lo];
-    spread_subproblem_mfma<W>(pr.zr, pr.grid, pr.g, dpoly, pr.pa.subs[gidx - prefix[lo]], shm);
+    const int4 sub = pr.pa.subs[gidx - prefix[lo]];
+    const MTile t = mfma_tile<W>(pr.g, sub.x);
+    const int e = mfma_strength_exp<MS::NW>(pr.zr, sub, red);
+    if (win.prob >= 0) {
+      const SpreadProblem &pw = probs[win.prob];
+      const int ncol = mfma_flush_cols<W>(win, lo, t, e, pw.g);
+      mfma_flush<W>(acc, pw.grid, pw.g, win.t, ncol, ldexpf(1.0f, win.e) * kProdUnscale);
+      if (ncol < MS::PX && e != win.e) mfma_rescale<W>(acc, ldexpf(1.0f, win.e - e));
+    }
+    win.prob = lo;
+    win.t = t;
+    win.e = e;
+    mfma_accumulate<W>(acc, pr.zr, dpoly, sub, t.xlo & (MS::PX - 1), ldexpf(1.0f, -e), shm);
+  }
+  if (win.prob >= 0) {
+    const SpreadProblem &pw = probs[win.prob];
+    mfma_flush<W>(acc, pw.grid, pw.g, win.t, MS::PX, ldexpf(1.0f, win.e) * kProdUnscale);
   }
 }
 
