@@ -231,6 +231,43 @@ struct TapPoly {
   }
 };
 
+// Tap polynomials staged in LDS as rows of kPolyRow floats per tap (the
+// kPolyCoefs coefficients, lowest degree first, zero padded), so that a lane
+// evaluates all taps of a point with broadcast reads of uniform addresses.
+constexpr int kPolyRow = 12;
+__device__ __forceinline__ void stage_poly_rows(const float (*__restrict__ coef)[kMaxTaps], int T,
+                                                float *rows) {
+  for (int k = threadIdx.x; k < T * kPolyRow; k += blockDim.x) {
+    const int tap = k / kPolyRow, d = k - tap * kPolyRow;
+    rows[k] = d < kPolyCoefs ? coef[d][tap] : 0.f;
+  }
+}
+
+// Weights of all 2W taps at two arguments z0, z1 (kb_taps's endpoint rule:
+// the first tap vanishes at z <= -1, the last at z >= 1).
+template <int W>
+__device__ __forceinline__ void taps2(const float *__restrict__ rows, float z0, float z1,
+                                      float (&v0)[2 * W], float (&v1)[2 * W]) {
+#pragma unroll
+  for (int i = 0; i < 2 * W; ++i) {
+    float c[kPolyRow];
+#pragma unroll
+    for (int d = 0; d < kPolyRow; d += 4) *(float4 *)(c + d) = *(const float4 *)(rows + i * kPolyRow + d);
+    float a = c[kPolyCoefs - 1], b = c[kPolyCoefs - 1];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) {
+      a = fmaf(a, z0, c[d]);
+      b = fmaf(b, z1, c[d]);
+    }
+    v0[i] = a;
+    v1[i] = b;
+  }
+  if (z0 <= -1.0f) v0[0] = 0.f;
+  if (z1 <= -1.0f) v1[0] = 0.f;
+  if (z0 >= 1.0f) v0[2 * W - 1] = 0.f;
+  if (z1 >= 1.0f) v1[2 * W - 1] = 0.f;
+}
+
 // LDS staging of the y and z weights of one point, zero-padded to the 32
 // column positions of the padded tile, so a lane reads its weight at its own
 // column index.
@@ -438,7 +475,8 @@ struct MLds {
   static constexpr int ZS = WZ + kMNP * kMWZ * 4;              // float [3][kMNP]
   static constexpr int OFS = ZS + 3 * kMNP * 4;                // int [kMNP]
   static constexpr int RED = OFS + kMNP * 4;                   // float [8]
-  static constexpr int SIZE = RED + 8 * 4;
+  static constexpr int POLY = RED + 8 * 4;                     // float [kMaxTaps][kPolyRow]
+  static constexpr int SIZE = POLY + kMaxTaps * kPolyRow * 4;
 };
 // two 4-wave workgroups per CU for w <= 7
 static_assert(2 * MLds<16>::SIZE <= 160 * 1024, "MFMA spreading LDS");
@@ -569,17 +607,37 @@ __device__ __forceinline__ void mfma_accumulate(f32x16 (&acc)[MSpread<W>::XW][2]
       C[p] = make_float2(c.x * s_in * kOpScale, c.y * s_in * kOpScale);
     }
     __syncthreads();
-    // pass 1: tap weights; each wave evaluates whole taps (coefficients in
-    // scalar registers, loaded once per tap) for all points and axes.  The
-    // x weights go to ring slot (tile column + base) mod PX.
-    for (int i0 = wave; i0 < T; i0 += NW) {
-      const TapPoly<W> tp(dpoly->c, __builtin_amdgcn_readfirstlane(i0));
-      const int i = tp.i;
-      for (int p = lane; p < np16; p += 64) {
-        const int off = OFS[p];
-        WX[(((off & 255) + i + base) & (PX - 1)) * kMWY + p] = tp(ZS[p]) * kOpScale;
-        WY[(((off >> 8) & 255) + i) * kMWY + p] = tp(ZS[kMNP + p]);
-        WZ[p * kMWZ + ((off >> 16) & 255) + i] = tp(ZS[2 * kMNP + p]);
+    // pass 1: tap weights; a task evaluates all taps of one axis for two
+    // points (p, p + np16 / 2).  The x weights go to ring slot (tile column
+    // + base) mod PX.
+    {
+      const float *PL = (const float *)(sh + L::POLY);
+      const int hn = np16 >> 1;
+      for (int u = threadIdx.x; u < 3 * hn; u += NT) {
+        const int a = (u >= hn) + (u >= 2 * hn);
+        const int p0 = u - a * hn, p1 = p0 + hn;
+        const int o0 = (OFS[p0] >> (8 * a)) & 255, o1 = (OFS[p1] >> (8 * a)) & 255;
+        float v0[T], v1[T];
+        taps2<W>(PL, ZS[a * kMNP + p0], ZS[a * kMNP + p1], v0, v1);
+        if (a == 0) {
+#pragma unroll
+          for (int i = 0; i < T; ++i) {
+            WX[((o0 + i + base) & (PX - 1)) * kMWY + p0] = v0[i] * kOpScale;
+            WX[((o1 + i + base) & (PX - 1)) * kMWY + p1] = v1[i] * kOpScale;
+          }
+        } else if (a == 1) {
+#pragma unroll
+          for (int i = 0; i < T; ++i) {
+            WY[(o0 + i) * kMWY + p0] = v0[i];
+            WY[(o1 + i) * kMWY + p1] = v1[i];
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < T; ++i) {
+            WZ[p0 * kMWZ + o0 + i] = v0[i];
+            WZ[p1 * kMWZ + o1 + i] = v1[i];
+          }
+        }
       }
     }
     __syncthreads();
@@ -759,6 +817,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
   for (int q = 0; q < MS::XW; ++q)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
+  stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
   MWindow win;
   const XcdChunk r(*pa.nsub);
   for (uint32_t sidx = r.begin; sidx < r.end; ++sidx) {
@@ -795,6 +854,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kerne
   for (int q = 0; q < MS::XW; ++q)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
+  stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
   MWindow win;
   const XcdChunk r(prefix[nprob]);
   for (uint32_t gidx = r.begin; gidx < r.end; ++gidx) {
@@ -955,7 +1015,8 @@ struct ILds {
   static constexpr int ZS = JIDX + kINP * 4;                   // float [3][kINP]
   static constexpr int OFS = ZS + 3 * kINP * 4;                // int [kINP]
   static constexpr int TMX = OFS + kINP * 4;                   // float [8]
-  static constexpr int SIZE = TMX + 8 * 4;
+  static constexpr int POLY = TMX + 8 * 4;                     // float [kMaxTaps][kPolyRow]
+  static constexpr int SIZE = POLY + kMaxTaps * kPolyRow * 4;
 };
 static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
 
@@ -975,6 +1036,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
   int *JIDX = (int *)(shm + L::JIDX);
   float *ZS = (float *)(shm + L::ZS);
   int *OFS = (int *)(shm + L::OFS);
+  float *PL = (float *)(shm + L::POLY);
+  stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first round's barriers
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
@@ -1057,16 +1120,22 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
         if (ok) JIDX[p] = (int)pa.perm[start + p];
       }
       __syncthreads();
-      // pass 1: tap weights; each wave evaluates whole taps (coefficients
-      // in scalar registers) for all points and axes
-      for (int i0 = wave; i0 < 2 * W; i0 += NW) {
-        const TapPoly<W> tp(dpoly->c, __builtin_amdgcn_readfirstlane(i0));
-        const int i = tp.i;
-        for (int p = lane; p < np32; p += 64) {
-          const int off = OFS[p];
-          WX[p * kIRow + (off & 255) + i] = tp(ZS[p]);
-          WY[p * kIRow + ((off >> 8) & 255) + i] = tp(ZS[kINP + p]);
-          WZ[p * kIRow + ((off >> 16) & 255) + i] = tp(ZS[2 * kINP + p]);
+      // pass 1: tap weights; a task evaluates all taps of one axis for two
+      // points (p, p + np32 / 2)
+      {
+        const int hn = np32 >> 1;
+        for (int u = threadIdx.x; u < 3 * hn; u += NT) {
+          const int a = (u >= hn) + (u >= 2 * hn);
+          const int p0 = u - a * hn, p1 = p0 + hn;
+          const int o0 = (OFS[p0] >> (8 * a)) & 255, o1 = (OFS[p1] >> (8 * a)) & 255;
+          float v0[2 * W], v1[2 * W];
+          taps2<W>(PL, ZS[a * kINP + p0], ZS[a * kINP + p1], v0, v1);
+          float *tab = a == 0 ? WX : (a == 1 ? WY : WZ);
+#pragma unroll
+          for (int t = 0; t < 2 * W; ++t) {
+            tab[p0 * kIRow + o0 + t] = v0[t];
+            tab[p1 * kIRow + o1 + t] = v1[t];
+          }
         }
       }
       __syncthreads();
