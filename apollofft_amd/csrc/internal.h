@@ -103,15 +103,27 @@ struct PointArgs {
   const uint32_t *nsub;      // device count of subproblems
 };
 
-// Compact per-point records in sorted order for the MFMA kernels (SoA):
-// z[a][s] = 1 - 2 (t - cell) of axis a (the argument of the tap
-// polynomials, FP64-folded and rounded once, exactly as kb_taps computes it),
-// off[s] = x register cell | y column << 8 | z column << 16 of the first taps
-// in the point's tile, c[s] = value (Type-1 only).
+// Compact per-point records in sorted order for the MFMA kernels, 32 bytes
+// per point as two float4: r[2s] = (z0, z1, z2, off), r[2s+1] = (c.x, c.y,
+// key, idx).  z[a] = 1 - 2 (t - cell) of axis a (the argument of the tap
+// polynomials, FP64-folded and rounded once, exactly as kb_taps computes
+// it); off = x register cell | y column << 8 | z column << 16 of the first
+// taps in the point's tile; c = value (Type-1, else 0); key = tile; idx =
+// input index of the point.  off, key and idx are stored as their bits.
 struct ZRec {
-  float *z[3];
-  int *off;
-  float2 *c;
+  float4 *r;
+};
+
+// Compact records built by the binning itself: the first digit pass reads
+// the coordinates and values in input order and scatters each point's record
+// (with its key and index) to its place; later passes move the records, and
+// the last one writes them to `out` (tmp is the other buffer of the
+// ping-pong).  The sorted keys and the permutation then live only in the
+// records: PointArgs::perm is not written.
+struct RecordOut {
+  ZRec out;
+  ZRec tmp;
+  const float2 *values;  // point values (Type-1) or null
 };
 
 // One problem of a batched spreading launch.
@@ -119,7 +131,7 @@ struct SpreadProblem {
   PointArgs pa;
   const float *rec;    // per-point records (launch_point_prep, vector-ALU kernel)
   const uint8_t *xo;   // per-point x offsets in the tile
-  ZRec zr;             // compact records (launch_point_z, MFMA kernel)
+  ZRec zr;             // compact records (built by sort_points, MFMA kernel)
   float2 *grid;
   Geom g;
 };
@@ -135,17 +147,15 @@ struct BatchChunk {
 hipError_t launch_fill(float2 *p, int64_t n, hipStream_t s);
 
 hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
-                       hipStream_t s);
+                       hipStream_t s, const RecordOut *rec = nullptr);
 
 hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
 
-// Per-point records in sorted order: point_rec_floats(w) floats (value and
-// the tap weights of the three axes with their tile offsets) and one x-offset
-// byte per point, filled by launch_point_prep (values may be NULL for
-// Type-2) and read by the spreading and MFMA interpolation kernels.
+// Per-point records in sorted order for the vector-ALU spreading kernel:
+// point_rec_floats(w) floats (value and the tap weights of the three axes
+// with their tile offsets) and one x-offset byte per point, filled by
+// launch_point_prep (values may be NULL for Type-2).
 int point_rec_floats(int w);
-hipError_t launch_point_z(const PointArgs &pa, const float2 *values, const Geom &g, ZRec zr,
-                          hipStream_t s);
 hipError_t launch_point_prep(int w, const PointArgs &pa, const float2 *values, const Geom &g,
                              const KbPoly &poly, float *rec, uint8_t *xo, hipStream_t s);
 hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,
@@ -200,6 +210,23 @@ __device__ __forceinline__ double fold_coord(float xf, double scale, int m, int 
   if (c < 0) c = 0;
   cell = c;
   return t;
+}
+
+// Compact record fields of point j (the ZRec layout): polynomial arguments
+// z[a] = 1 - 2 (t - cell) and the tile offsets of the first taps (x: cell mod
+// bin; y, z: cell mod bin + 1, the tile starting w cells before its bin).
+__device__ __forceinline__ void point_zrec(const PointArgs &pa, const Geom &g, int64_t j,
+                                           float (&z)[3], int &off) {
+  off = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float *cp = a == 0 ? pa.x : (a == 1 ? pa.y : pa.z);
+    int cell;
+    const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], cell);
+    z[a] = (float)(1.0 - 2.0 * (t - (double)cell));
+    const int o = cell - (cell / g.bin[a]) * g.bin[a];
+    off |= (a == 0 ? o : o + 1) << (8 * a);
+  }
 }
 
 // Evaluate the 2w tap weights of one axis for grid coordinate t in cell c

@@ -182,7 +182,7 @@ struct StageTimer {
 
 struct Layout {
   size_t keys0 = 0, keys1 = 0, idx0 = 0, idx1 = 0, cnt = 0, scan = 0, bin_start = 0,
-         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, tmax = 0, grid = 0, fftbuf = 0,
+         sub_ofs = 0, subs = 0, rec = 0, xo = 0, zrec = 0, zrec2 = 0, tmax = 0, grid = 0, fftbuf = 0,
          total = 0;
   int nchunks = 0;
 };
@@ -268,7 +268,8 @@ Layout layout(const anufft_plan *p, int64_t M) {
       nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
     l.subs = take(16 * (size_t)nsub);
     if (p->kernel_method == kSpreadMfma) {
-      l.zrec = take(24 * Mp);  // z[3], off (4 B each), value (8 B) per point
+      l.zrec = take(32 * Mp);   // ZRec: 32-byte record per point
+      l.zrec2 = take(32 * Mp);  // the other record buffer of the binning passes
       l.tmax = take(2 * sizeof(float) * (size_t)(p->bi.nbins + 1));  // bin and tile maxima
     }
     else
@@ -632,29 +633,33 @@ struct Exec {
   float2 *grid() const { return (float2 *)(ws + lay.grid); }
   float *rec() const { return (float *)(ws + lay.rec); }
   uint8_t *xo() const { return (uint8_t *)(ws + lay.xo); }
-  ZRec zrec(int64_t M) const {
-    const size_t Mp = (size_t)std::max<int64_t>(M, 1);
+  ZRec zrec(int64_t, bool second = false) const {
     ZRec z{};
-    float *b = (float *)(ws + lay.zrec);
-    z.z[0] = b;
-    z.z[1] = b + Mp;
-    z.z[2] = b + 2 * Mp;
-    z.off = (int *)(b + 3 * Mp);
-    z.c = (float2 *)(b + 4 * Mp);
+    z.r = (float4 *)(ws + (second ? lay.zrec2 : lay.zrec));
     return z;
   }
-  // Per-point records of sorted points: compact ones for the MFMA kernels,
-  // full weight records for the vector-ALU spreading kernel.
+  // Binning; with records (MFMA kernels) the sort passes also build the
+  // compact per-point records in sorted order.
+  hipError_t bin(const anufft_plan::Binning &b, const PointArgs &pa, const float2 *values,
+                 bool records) const {
+    return timed(ANUFFT_STAGE_BIN, [&] {
+      if (!records) return sort_points(pa, b.g, 3, sort_buffers(b), s);
+      const RecordOut ro{zrec(pa.num_points), zrec(pa.num_points, true), values};
+      return sort_points(pa, b.g, 3, sort_buffers(b), s, &ro);
+    });
+  }
+  // Full weight records of sorted points for the vector-ALU spreading kernel.
   hipError_t prep(const anufft_plan::Binning &b, const PointArgs &pa, const float2 *values) const {
     return timed(ANUFFT_STAGE_PREP, [&] {
-      if (p->kernel_method == kSpreadMfma) return launch_point_z(pa, values, b.g, zrec(pa.num_points), s);
       return launch_point_prep(p->w, pa, values, b.g, p->poly, rec(), xo(), s);
     });
   }
-  // Type-1 spreading of sorted points: records, then the tile kernel.
+  // Type-1 spreading of binned points (records, then the tile kernel).
   hipError_t spread3(const PointArgs &pa, const float2 *values, float2 *grid) const {
-    hipError_t e = prep(p->bs, pa, values);
-    if (e != hipSuccess) return e;
+    if (p->kernel_method != kSpreadMfma) {
+      hipError_t e = prep(p->bs, pa, values);
+      if (e != hipSuccess) return e;
+    }
     return timed(ANUFFT_STAGE_SPREAD, [&] {
       return launch_spread3d(p->w, p->kernel_method, pa, rec(), xo(), zrec(pa.num_points), grid,
                              p->bs.g, p->poly, p->d_poly, blocks(p->bs, pa.num_points, 2), s);
@@ -910,7 +915,7 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   if (M > 0) {
     if (p->dim == 3) {
       PointArgs pa = ex.point_args(p->bs, M, coords, stride);
-      e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bs.g, 3, ex.sort_buffers(p->bs), ex.s); });
+      e = ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma);
       if (e != hipSuccess) return hip_fail(e, "binning");
       e = ex.spread3(pa, (const float2 *)values, ex.grid());
     } else {
@@ -938,12 +943,8 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
   hipError_t e;
   if (p->dim == 3) {
     PointArgs pa = ex.point_args(p->bi, M, coords, stride);
-    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, p->bi.g, 3, ex.sort_buffers(p->bi), ex.s); });
+    e = ex.bin(p->bi, pa, nullptr, p->kernel_method == kSpreadMfma);
     if (e != hipSuccess) return hip_fail(e, "binning");
-    if (p->kernel_method == kSpreadMfma) {
-      e = ex.prep(p->bi, pa, nullptr);
-      if (e != hipSuccess) return hip_fail(e, "point records");
-    }
     e = ex.timed(ANUFFT_STAGE_FFT, [&] {
       return ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
     });
@@ -983,12 +984,8 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
     const bool gm = grad_uses_mfma(p->w, p->kernel_method);
     const anufft_plan::Binning &b = gm ? p->bi : p->bg;
     PointArgs pa = ex.point_args(b, M, coords, stride);
-    e = ex.timed(ANUFFT_STAGE_BIN, [&] { return sort_points(pa, b.g, 3, ex.sort_buffers(b), ex.s); });
+    e = ex.bin(b, pa, nullptr, gm);
     if (e != hipSuccess) return hip_fail(e, "binning");
-    if (gm) {
-      e = ex.prep(b, pa, nullptr);
-      if (e != hipSuccess) return hip_fail(e, "point records");
-    }
     e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
     if (e != hipSuccess) return hip_fail(e, "fft");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
@@ -1084,7 +1081,7 @@ anufft_status anufft_debug_spread(const anufft_plan *p, int64_t M, const float *
   if (M == 0) return ANUFFT_OK;
   if (p->dim == 3) {
     PointArgs pa = ex.point_args(p->bs, M, coords, stride);
-    e = sort_points(pa, p->bs.g, 3, ex.sort_buffers(p->bs), ex.s);
+    e = ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma);
     if (e != hipSuccess) return hip_fail(e, "binning");
     e = ex.spread3(pa, (const float2 *)values, (float2 *)grid);
   } else {
@@ -1184,27 +1181,28 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
     return hipSuccess;
   });
   if (e != hipSuccess) return hip_fail(e, "fill");
+  const bool mf = p0->kernel_method == kSpreadMfma;
   e = e0.timed(ANUFFT_STAGE_BIN, [&] {
     for (int i = 0; i < nprob; ++i) {
-      hipError_t r = sort_points(probs[i].pa, plans[i]->bs.g, 3, ex[i].sort_buffers(plans[i]->bs), s);
+      const RecordOut ro{probs[i].zr, ex[i].zrec(num_points[i], true), (const float2 *)values[i]};
+      hipError_t r = sort_points(probs[i].pa, plans[i]->bs.g, 3, ex[i].sort_buffers(plans[i]->bs),
+                                 s, mf ? &ro : nullptr);
       if (r != hipSuccess) return r;
     }
     return hipSuccess;
   });
   if (e != hipSuccess) return hip_fail(e, "binning");
-  e = e0.timed(ANUFFT_STAGE_PREP, [&] {
-    for (int i = 0; i < nprob; ++i) {
-      hipError_t r = p0->kernel_method == kSpreadMfma
-                         ? launch_point_z(probs[i].pa, (const float2 *)values[i], plans[i]->bs.g,
-                                          probs[i].zr, s)
-                         : launch_point_prep(p0->w, probs[i].pa, (const float2 *)values[i],
-                                             plans[i]->bs.g, plans[i]->poly, ex[i].rec(),
-                                             ex[i].xo(), s);
-      if (r != hipSuccess) return r;
-    }
-    return hipSuccess;
-  });
-  if (e != hipSuccess) return hip_fail(e, "spread records");
+  if (!mf) {
+    e = e0.timed(ANUFFT_STAGE_PREP, [&] {
+      for (int i = 0; i < nprob; ++i) {
+        hipError_t r = launch_point_prep(p0->w, probs[i].pa, (const float2 *)values[i],
+                                         plans[i]->bs.g, plans[i]->poly, ex[i].rec(), ex[i].xo(), s);
+        if (r != hipSuccess) return r;
+      }
+      return hipSuccess;
+    });
+    if (e != hipSuccess) return hip_fail(e, "spread records");
+  }
   const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>(sub_upper, 2 * (int64_t)p0->cus));
   e = e0.timed(ANUFFT_STAGE_SPREAD, [&] {
     return launch_spread3d_batched(p0->w, p0->kernel_method, probs.data(), nprob, table, prefix,

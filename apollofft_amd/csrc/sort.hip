@@ -57,10 +57,18 @@ __global__ __launch_bounds__(kSortThreads) void keys_kernel(PointArgs pa, Geom g
     cnt[(int64_t)d * nchunks + blockIdx.x] = hist[d];
 }
 
+// Keys of the current order: a key array, or the key field of the records.
+struct KeySrc {
+  const uint32_t *k;
+  const float4 *r;
+  __device__ __forceinline__ uint32_t operator[](int64_t j) const {
+    return r ? __float_as_uint(r[2 * j + 1].z) : k[j];
+  }
+};
+
 // Per-chunk histogram of digit `shift` (passes after the first).
-__global__ __launch_bounds__(kSortThreads) void hist_kernel(const uint32_t *keys, int64_t n,
-                                                            int shift, uint32_t *cnt,
-                                                            int nchunks) {
+__global__ __launch_bounds__(kSortThreads) void hist_kernel(KeySrc keys, int64_t n, int shift,
+                                                            uint32_t *cnt, int nchunks) {
   __shared__ uint32_t hist[kRadix];
   for (int d = threadIdx.x; d < kRadix; d += kSortThreads) hist[d] = 0;
   __syncthreads();
@@ -156,9 +164,22 @@ hipError_t scan_exclusive(uint32_t *data, int64_t n, uint32_t *tmp, hipStream_t 
 // One block per chunk; the chunk is walked in input order, kSortThreads points
 // per step.  Ranks inside a wave come from matching digits with per-bit
 // ballots; ranks across the waves of a step from per-wave digit counts.
+// With REC the pass moves 32-byte records instead of separate keys and
+// indices: the first pass builds each point's record from its coordinates
+// and value (keys_in holds the keys of the input order), later passes read
+// the records of the current order from `in`; all write to `out`.
+struct RecMove {
+  ZRec in, out;
+  const float2 *values;  // value field present (Type-1) or null
+  PointArgs pa;
+  Geom g;
+  int first;
+};
+
+template <bool REC>
 __global__ __launch_bounds__(kSortThreads) void scatter_kernel(
     const uint32_t *keys_in, const uint32_t *idx_in, uint32_t *keys_out, uint32_t *idx_out,
-    int64_t n, int shift, const uint32_t *offs, int nchunks) {
+    int64_t n, int shift, const uint32_t *offs, int nchunks, RecMove rm) {
   constexpr int kWaves = kSortThreads / 64;
   __shared__ uint32_t running[kRadix];
   __shared__ uint32_t wcount[kWaves][kRadix];
@@ -173,8 +194,27 @@ __global__ __launch_bounds__(kSortThreads) void scatter_kernel(
   for (int step = 0; step < kSortItems; ++step) {
     int64_t j = base + (int64_t)step * kSortThreads + tid;
     bool valid = j < n;
-    uint32_t key = valid ? keys_in[j] : 0u;
-    uint32_t idx = valid ? (idx_in ? idx_in[j] : (uint32_t)j) : 0u;
+    uint32_t key = 0u, idx = 0u;
+    float4 ra = make_float4(0.f, 0.f, 0.f, 0.f), rb = ra;
+    if (!REC) {
+      key = valid ? keys_in[j] : 0u;
+      idx = valid ? (idx_in ? idx_in[j] : (uint32_t)j) : 0u;
+    } else if (valid) {
+      if (rm.first) {
+        key = keys_in[j];
+        idx = (uint32_t)j;
+        float z[3];
+        int off;
+        point_zrec(rm.pa, rm.g, j, z, off);
+        const float2 c = rm.values ? rm.values[j] : make_float2(0.f, 0.f);
+        ra = make_float4(z[0], z[1], z[2], __int_as_float(off));
+        rb = make_float4(c.x, c.y, __uint_as_float(key), __uint_as_float(idx));
+      } else {
+        ra = rm.in.r[2 * j];
+        rb = rm.in.r[2 * j + 1];
+        key = __float_as_uint(rb.z);
+      }
+    }
     uint32_t d = (key >> shift) & (kRadix - 1);
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -189,8 +229,13 @@ __global__ __launch_bounds__(kSortThreads) void scatter_kernel(
     if (valid) {
       uint32_t pos = running[d] + rank;
       for (int w = 0; w < wave; ++w) pos += wcount[w][d];
-      keys_out[pos] = key;
-      idx_out[pos] = idx;
+      if (REC) {
+        rm.out.r[2 * (int64_t)pos] = ra;
+        rm.out.r[2 * (int64_t)pos + 1] = rb;
+      } else {
+        keys_out[pos] = key;
+        idx_out[pos] = idx;
+      }
     }
     __syncthreads();
     for (int dd = tid; dd < kRadix; dd += kSortThreads) {
@@ -206,7 +251,7 @@ __global__ __launch_bounds__(kSortThreads) void scatter_kernel(
 }
 
 // bin_start[b] = first sorted position with key >= b, b in [0, nbins].
-__global__ void bounds_kernel(const uint32_t *keys, int64_t n, uint32_t *bin_start, int nbins) {
+__global__ void bounds_kernel(KeySrc keys, int64_t n, uint32_t *bin_start, int nbins) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > nbins) return;
   int64_t lo = 0, hi = n;
@@ -256,24 +301,41 @@ __global__ void subwrite_kernel(const uint32_t *bin_start, const uint32_t *sub_o
 } // namespace
 
 hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
-                       hipStream_t s) {
+                       hipStream_t s, const RecordOut *rec) {
   const int64_t n = pa.num_points;
+  // with records, the last pass writes rec->out; earlier ones alternate with tmp
+  auto dst = [&](int q) { return ((sb.passes - 1 - q) & 1) == 0 ? rec->out : rec->tmp; };
   if (n > 0) {
     hipLaunchKernelGGL(keys_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s, pa, g, dim,
                        sb.keys[0], sb.cnt, sb.nchunks);
     for (int p = 0; p < sb.passes; ++p) {
-      if (p > 0)
-        hipLaunchKernelGGL(hist_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
-                           sb.keys[p & 1], n, p * kRadixBits, sb.cnt, sb.nchunks);
+      if (p > 0) {
+        const KeySrc ks{sb.keys[p & 1], rec ? dst(p - 1).r : nullptr};
+        hipLaunchKernelGGL(hist_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s, ks, n,
+                           p * kRadixBits, sb.cnt, sb.nchunks);
+      }
       hipError_t e = scan_exclusive(sb.cnt, (int64_t)kRadix * sb.nchunks, sb.scan_tmp, s);
       if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(scatter_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
-                         sb.keys[p & 1], p == 0 ? (const uint32_t *)nullptr : sb.idx[p & 1],
-                         sb.keys[(p + 1) & 1], sb.idx[(p + 1) & 1], n, p * kRadixBits, sb.cnt,
-                         sb.nchunks);
+      RecMove rm{};
+      if (rec) {
+        rm.out = dst(p);
+        if (p > 0) rm.in = dst(p - 1);
+        rm.values = rec->values;
+        rm.pa = pa;
+        rm.g = g;
+        rm.first = p == 0;
+        hipLaunchKernelGGL(scatter_kernel<true>, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
+                           sb.keys[0], (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                           (uint32_t *)nullptr, n, p * kRadixBits, sb.cnt, sb.nchunks, rm);
+      } else {
+        hipLaunchKernelGGL(scatter_kernel<false>, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
+                           sb.keys[p & 1], p == 0 ? (const uint32_t *)nullptr : sb.idx[p & 1],
+                           sb.keys[(p + 1) & 1], sb.idx[(p + 1) & 1], n, p * kRadixBits, sb.cnt,
+                           sb.nchunks, rm);
+      }
     }
   }
-  const uint32_t *sorted = sb.keys[sb.passes & 1];
+  const KeySrc sorted{sb.keys[sb.passes & 1], rec ? rec->out.r : nullptr};
   int nb1 = sb.nbins + 1;
   const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
   hipLaunchKernelGGL(bounds_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sorted, n,

@@ -187,29 +187,6 @@ __global__ __launch_bounds__(256) void point_prep_kernel(PointArgs pa, const flo
   }
 }
 
-// Compact records for the MFMA kernels: per sorted point the three tap
-// polynomial arguments z = 1 - 2 (t - cell), the tile offsets of the first
-// taps and the value.
-__global__ __launch_bounds__(256) void point_z_kernel(PointArgs pa, const float2 *values, Geom g,
-                                                      ZRec zr) {
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < pa.num_points; s += st) {
-    const int64_t j = (int64_t)pa.perm[s];
-    int off = 0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float *cp = a == 0 ? pa.x : (a == 1 ? pa.y : pa.z);
-      int cell;
-      const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], cell);
-      zr.z[a][s] = (float)(1.0 - 2.0 * (t - (double)cell));
-      const int o = cell - (cell / g.bin[a]) * g.bin[a];
-      off |= (a == 0 ? o : o + 1) << (8 * a);
-    }
-    zr.off[s] = off;
-    if (values) zr.c[s] = values[j];
-  }
-}
-
 // The polynomial of one tap (wave-uniform index i), coefficients read once
 // with scalar loads; value at argument z with kb_taps's endpoint rule.
 template <int W>
@@ -548,7 +525,7 @@ __device__ __forceinline__ int mfma_strength_exp(const ZRec &zr, int4 sub, float
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   float mx = 0.f;
   for (int p = threadIdx.x; p < sub.z; p += 64 * NW) {
-    const float2 c = zr.c[sub.y + p];
+    const float4 c = zr.r[2 * (int64_t)(sub.y + p) + 1];
     mx = fmaxf(mx, fmaxf(fabsf(c.x), fabsf(c.y)));
   }
   mx = wave_max_nonneg(mx);
@@ -600,10 +577,12 @@ __device__ __forceinline__ void mfma_accumulate(f32x16 (&acc)[MSpread<W>::XW][2]
 #pragma unroll
       for (int zz = 0; zz < 32; ++zz) WZ[p * kMWZ + zz] = 0.f;
       const bool ok = p < npts;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) ZS[a * kMNP + p] = ok ? zr.z[a][start + p] : 0.f;
-      OFS[p] = ok ? zr.off[start + p] : 0;
-      const float2 c = ok ? zr.c[start + p] : make_float2(0.f, 0.f);
+      const float4 z = ok ? zr.r[2 * (int64_t)(start + p)] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 c = ok ? zr.r[2 * (int64_t)(start + p) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+      ZS[p] = z.x;
+      ZS[kMNP + p] = z.y;
+      ZS[2 * kMNP + p] = z.z;
+      OFS[p] = __float_as_int(z.w);
       C[p] = make_float2(c.x * s_in * kOpScale, c.y * s_in * kOpScale);
     }
     __syncthreads();
@@ -1114,10 +1093,12 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
           *(float4 *)(WZ + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         const bool ok = p < npts;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ZS[a * kINP + p] = ok ? zr.z[a][start + p] : 0.f;
-        OFS[p] = ok ? zr.off[start + p] : 0;
-        if (ok) JIDX[p] = (int)pa.perm[start + p];
+        const float4 z = ok ? zr.r[2 * (int64_t)(start + p)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        ZS[p] = z.x;
+        ZS[kINP + p] = z.y;
+        ZS[2 * kINP + p] = z.z;
+        OFS[p] = __float_as_int(z.w);
+        if (ok) JIDX[p] = (int)__float_as_uint(zr.r[2 * (int64_t)(start + p) + 1].w);
       }
       __syncthreads();
       // pass 1: tap weights; a task evaluates all taps of one axis for two
@@ -1324,10 +1305,12 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
           for (int i = 0; i < 32; i += 4) *(float4 *)(row(r, p) + i) = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         const bool ok = p < npts;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) ZS[a * kGNP + p] = ok ? zr.z[a][start + p] : 0.f;
-        OFS[p] = ok ? zr.off[start + p] : 0;
-        if (ok) JIDX[p] = (int)pa.perm[start + p];
+        const float4 z = ok ? zr.r[2 * (int64_t)(start + p)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        ZS[p] = z.x;
+        ZS[kGNP + p] = z.y;
+        ZS[2 * kGNP + p] = z.z;
+        OFS[p] = __float_as_int(z.w);
+        if (ok) JIDX[p] = (int)__float_as_uint(zr.r[2 * (int64_t)(start + p) + 1].w);
       }
       __syncthreads();
       // pass 1: value and derivative taps, whole taps per wave
@@ -1922,15 +1905,6 @@ int spread_bin_x(int w, int method) {
 int spread_block_threads(int w, int method) {
   if (method != kSpreadMfma) return kSpreadThreads;
   return w >= 8 ? 512 : 256;
-}
-
-hipError_t launch_point_z(const PointArgs &pa, const float2 *values, const Geom &g, ZRec zr,
-                          hipStream_t s) {
-  if (pa.num_points <= 0) return hipSuccess;
-  int64_t blocks = (pa.num_points + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(point_z_kernel, dim3((unsigned)blocks), dim3(256), 0, s, pa, values, g, zr);
-  return hipGetLastError();
 }
 
 hipError_t launch_spread3d(int w, int method, const PointArgs &pa, const float *rec,

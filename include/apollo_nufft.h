@@ -215,12 +215,13 @@ anufft_status anufft_execute_type2_host(const anufft_plan *plan, int32_t storage
  * the records.  Not for concurrent use of one plan from several threads. */
 typedef enum anufft_stage {
   ANUFFT_STAGE_FILL = 0,   /* zeroing of the oversampled grid */
-  ANUFFT_STAGE_BIN = 1,    /* point binning (counting sort) and subproblems */
+  ANUFFT_STAGE_BIN = 1,    /* point binning (counting sort) and subproblems; with the
+                              MFMA kernels also the per-point records */
   ANUFFT_STAGE_SPREAD = 2, /* Type-1 spreading */
   ANUFFT_STAGE_FFT = 3,    /* all FFT passes with pad/crop/deconvolution */
   ANUFFT_STAGE_INTERP = 4, /* Type-2 interpolation */
   ANUFFT_STAGE_GRAD = 5,   /* coordinate-gradient interpolation */
-  ANUFFT_STAGE_PREP = 6,   /* per-point spreading records (value, x-weights) */
+  ANUFFT_STAGE_PREP = 6,   /* per-point weight records (vector-ALU kernels) */
   ANUFFT_STAGE_COUNT = 7
 } anufft_stage;
 anufft_status anufft_timing_enable(const anufft_plan *plan, int32_t on);
