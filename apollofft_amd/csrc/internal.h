@@ -56,7 +56,7 @@ struct SortBuffers {
   uint32_t *cnt;       // kRadix * nchunks digit counts, then offsets
   uint32_t *scan_tmp;  // block sums of the scan
   uint32_t *bin_start; // nbins + 1
-  uint32_t *sub_ofs;   // nbins + 1 (exclusive scan of subproblem counts)
+  uint32_t *sub_ofs;   // nbins + 1; sub_ofs[nbins] = number of subproblems
   int4 *subs;          // (bin, start, count, 0)
   int nbins;
   int nchunks;

@@ -11,6 +11,7 @@
 //   keys   : key[j] = tile of point j, per-chunk histogram of digit 0
 //   per digit pass: scan(counts) -> stable scatter (-> histogram of next digit)
 //   bounds : bin_start[b] = lower_bound(sorted keys, b)
+//   subproblems (one block): scan of per-tile subproblem counts, list
 //   subs   : subproblems of at most max_sub points per tile, tiles in pencil
 //            order (x fastest)
 #include "internal.h"
@@ -83,20 +84,21 @@ __global__ __launch_bounds__(kSortThreads) void hist_kernel(KeySrc keys, int64_t
 }
 
 // ---- exclusive scan of a uint32 array (reduce, scan block sums, apply) ----
+template <int NT = kScanThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
                                                          uint32_t &total) {
-  // Hillis-Steele over the block in LDS; kScanThreads entries.
+  // Hillis-Steele over the block in LDS; NT entries.
   int t = threadIdx.x;
   sh[t] = v;
   __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
+  for (int off = 1; off < NT; off <<= 1) {
     uint32_t add = t >= off ? sh[t - off] : 0u;
     __syncthreads();
     sh[t] += add;
     __syncthreads();
   }
   uint32_t incl = sh[t];
-  total = sh[kScanThreads - 1];
+  total = sh[NT - 1];
   __syncthreads();
   return incl - v;
 }
@@ -115,23 +117,15 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of the block sums, any length.
-__global__ __launch_bounds__(kScanThreads) void scan_sums_kernel(uint32_t *sums, int64_t n) {
-  __shared__ uint32_t sh[kScanThreads];
-  uint32_t carry = 0;
-  for (int64_t base = 0; base < n; base += kScanThreads) {
-    int64_t j = base + threadIdx.x;
-    uint32_t v = j < n ? sums[j] : 0u;
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan(v, sh, total);
-    if (j < n) sums[j] = ex + carry;
-    carry += total;
-  }
-}
-
+// Exclusive scan of each tile, offset by the sum of the tile sums before it
+// (read by every block: the tile count stays small, a few thousand at most).
 __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *data, int64_t n,
                                                                   const uint32_t *sums) {
   __shared__ uint32_t sh[kScanThreads];
+  uint32_t before = 0;
+  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) before += sums[i];
+  uint32_t boff;
+  block_exclusive_scan(before, sh, boff);
   int64_t base = (int64_t)blockIdx.x * kScanTile;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -141,7 +135,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *data
     s += v[i];
   }
   uint32_t total;
-  uint32_t ex = block_exclusive_scan(s, sh, total) + sums[blockIdx.x];
+  uint32_t ex = block_exclusive_scan(s, sh, total) + boff;
   for (int i = 0; i < kScanItems; ++i) {
     int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
     if (j < n) data[j] = ex;
@@ -154,7 +148,6 @@ hipError_t scan_exclusive(uint32_t *data, int64_t n, uint32_t *tmp, hipStream_t 
   int64_t nb = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
                      tmp);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(kScanThreads), 0, s, tmp, nb);
   hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
                      tmp);
   return hipGetLastError();
@@ -271,31 +264,34 @@ __device__ __forceinline__ int pencil_bin(int q, int3 nb) {
   return (bx * nb.y + pz / nb.z) * nb.z + pz % nb.z;
 }
 
-__global__ void subcount_kernel(const uint32_t *bin_start, int nbins, int3 nb, int max_sub,
-                                uint32_t *sub_ofs) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q > nbins) return;
-  if (q == nbins) {
-    sub_ofs[q] = 0;
-    return;
+// Subproblem list in one block: tiles in pencil order, ceil(count / max_sub)
+// subproblems each, exclusive scan of those counts in rounds of kSubThreads
+// tiles; the total goes to *nsub.
+constexpr int kSubThreads = 1024;
+__global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t *bin_start,
+                                                                  int nbins, int3 nb, int max_sub,
+                                                                  uint32_t *nsub, int4 *subs) {
+  __shared__ uint32_t sh[kSubThreads];
+  uint32_t carry = 0;
+  for (int base = 0; base < nbins; base += kSubThreads) {
+    const int q = base + threadIdx.x;
+    int b = 0;
+    uint32_t start = 0, c = 0, k = 0;
+    if (q < nbins) {
+      b = pencil_bin(q, nb);
+      start = bin_start[b];
+      c = bin_start[b + 1] - start;
+      k = (c + (uint32_t)max_sub - 1) / (uint32_t)max_sub;
+    }
+    uint32_t total;
+    const uint32_t o = carry + block_exclusive_scan<kSubThreads>(k, sh, total);
+    for (uint32_t t = 0; t < k; ++t) {
+      const uint32_t cnt = min(c - t * (uint32_t)max_sub, (uint32_t)max_sub);
+      subs[o + t] = make_int4(b, (int)(start + t * (uint32_t)max_sub), (int)cnt, 0);
+    }
+    carry += total;
   }
-  const int b = pencil_bin(q, nb);
-  uint32_t c = bin_start[b + 1] - bin_start[b];
-  sub_ofs[q] = (c + (uint32_t)max_sub - 1) / (uint32_t)max_sub;
-}
-
-__global__ void subwrite_kernel(const uint32_t *bin_start, const uint32_t *sub_ofs, int nbins,
-                                int3 nb, int max_sub, int4 *subs) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nbins) return;
-  const int b = pencil_bin(q, nb);
-  uint32_t start = bin_start[b], c = bin_start[b + 1] - start;
-  uint32_t o = sub_ofs[q];
-  for (uint32_t s = 0; s * (uint32_t)max_sub < c; ++s) {
-    uint32_t cnt = c - s * (uint32_t)max_sub;
-    if (cnt > (uint32_t)max_sub) cnt = (uint32_t)max_sub;
-    subs[o + s] = make_int4(b, (int)(start + s * (uint32_t)max_sub), (int)cnt, 0);
-  }
+  if (threadIdx.x == 0) *nsub = carry;
 }
 
 } // namespace
@@ -340,12 +336,8 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
   const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
   hipLaunchKernelGGL(bounds_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sorted, n,
                      sb.bin_start, sb.nbins);
-  hipLaunchKernelGGL(subcount_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sb.bin_start,
-                     sb.nbins, nb, sb.max_sub, sb.sub_ofs);
-  hipError_t e = scan_exclusive(sb.sub_ofs, nb1, sb.scan_tmp, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(subwrite_kernel, dim3((sb.nbins + 255) / 256), dim3(256), 0, s,
-                     sb.bin_start, sb.sub_ofs, sb.nbins, nb, sb.max_sub, sb.subs);
+  hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
+                     sb.nbins, nb, sb.max_sub, sb.sub_ofs + sb.nbins, sb.subs);
   return hipGetLastError();
 }
 
