@@ -146,6 +146,32 @@ def test_type2_3d_vs_direct(dev, oracle_mod, n, w, policy, method):
     assert err < max(TOL, 2.0 * kb_err), (err, kb_err)
 
 
+def test_three_digit_binning_with_records(dev, oracle_mod):
+    """A 512^3 grid with w = 7 has more than 65536 spreading tiles, so the
+    binning takes three digit passes and the point records go out -> tmp ->
+    out; Type-1 at sampled modes and Type-2 at a few points against FP64
+    direct sums."""
+    import apollofft_amd._native as N
+    n, w = (256, 256, 256), 7
+    plan, d = _plan3(n, w=w, method="mfma")
+    nb = ctypes.c_int64()
+    N.check(N.lib().anufft_debug_num_bins(plan.handle, ctypes.byref(nb)))
+    assert nb.value > 65536
+    rng = np.random.default_rng(31)
+    M = 3000
+    xyz = uniform_points(rng, M)
+    c = cnormal(rng, M)
+    out = plan.type1(_t(xyz, dev), _t(c, dev)).cpu().numpy().ravel()
+    sel = rng.choice(int(np.prod(n)), size=400, replace=False).astype(np.int64)
+    ref = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d, sel)
+    assert rel_l2(out[sel], ref) < TOL
+    f = cnormal(rng, n)
+    xyz2 = xyz[:200]
+    out2 = plan.type2(_t(xyz2, dev), _t(f, dev)).cpu().numpy()
+    ref2 = oracle_mod.direct_type2_3d(f.astype(np.complex128), xyz2.astype(np.float64), d)
+    assert rel_l2(out2, ref2) < TOL
+
+
 def test_type2_3d_compat_scaling(dev, oracle_mod):
     """compat flag reproduces the reference's direct/(mx my mz) output
     (dimension_3d.rs:549-551)."""
