@@ -50,14 +50,20 @@ struct Geom {
   double scale[3];   // m / L: grid units per unit length
 };
 
+// Work of a subproblem for splitting the list among workgroups: one unit
+// per kWorkUnit points plus kSubWork units of fixed cost (staging, scale,
+// flush).
+constexpr uint32_t kWorkUnit = 16;
+constexpr uint32_t kSubWork = 16;
+
 struct SortBuffers {
   uint32_t *keys[2];
   uint32_t *idx[2];
   uint32_t *cnt;       // kRadix * nchunks digit counts, then offsets
   uint32_t *scan_tmp;  // block sums of the scan
   uint32_t *bin_start; // nbins + 1
-  uint32_t *sub_ofs;   // nbins + 1; sub_ofs[nbins] = number of subproblems
-  int4 *subs;          // (bin, start, count, 0)
+  uint32_t *sub_ofs;   // nbins + 2; [nbins] = number of subproblems, [nbins + 1] = total work
+  int4 *subs;          // (bin, start, count, work before it)
   int nbins;
   int nchunks;
   int passes;
@@ -100,7 +106,7 @@ struct PointArgs {
   int64_t num_points;
   const uint32_t *perm;      // sorted order -> original point index
   const int4 *subs;
-  const uint32_t *nsub;      // device count of subproblems
+  const uint32_t *nsub;      // device: [0] subproblems, [1] their total work (subs[i].w units)
 };
 
 // Compact per-point records in sorted order for the MFMA kernels, 32 bytes

@@ -262,7 +262,7 @@ Layout layout(const anufft_plan *p, int64_t M) {
     int64_t scan_n = std::max<int64_t>(cnt_n, nb + 1);
     l.scan = take(4 * (size_t)((scan_n + 1023) / 1024 + 1));
     l.bin_start = take(4 * (size_t)(nb + 1));
-    l.sub_ofs = take(4 * (size_t)(nb + 1));
+    l.sub_ofs = take(4 * (size_t)(nb + 2));
     int64_t nsub = 1;
     for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
       nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
@@ -1103,7 +1103,7 @@ anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32
     tot += align_up(layout(plans[i], max_points[i]).total);
   }
   tot += align_up(sizeof(SpreadProblem) * (size_t)std::max(nprob, 1));
-  tot += align_up(sizeof(uint32_t) * (size_t)(nprob + 1));
+  tot += align_up(sizeof(uint32_t) * (size_t)(2 * nprob + 2));  // subproblem and work prefixes
   *bytes = tot;
   return ANUFFT_OK;
 }

@@ -266,32 +266,43 @@ __device__ __forceinline__ int pencil_bin(int q, int3 nb) {
 
 // Subproblem list in one block: tiles in pencil order, ceil(count / max_sub)
 // subproblems each, exclusive scan of those counts in rounds of kSubThreads
-// tiles; the total goes to *nsub.
+// tiles.  subs[i].w = work before subproblem i in list order, in units of
+// kWorkUnit points, a subproblem weighing ceil(count / kWorkUnit) +
+// kSubWork (its fixed cost); nsub[0] = number of subproblems, nsub[1] =
+// total work.
 constexpr int kSubThreads = 1024;
 __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t *bin_start,
                                                                   int nbins, int3 nb, int max_sub,
                                                                   uint32_t *nsub, int4 *subs) {
   __shared__ uint32_t sh[kSubThreads];
-  uint32_t carry = 0;
+  const uint32_t ms = (uint32_t)max_sub;
+  uint32_t carry = 0, carry_w = 0;
   for (int base = 0; base < nbins; base += kSubThreads) {
     const int q = base + threadIdx.x;
     int b = 0;
-    uint32_t start = 0, c = 0, k = 0;
+    uint32_t start = 0, c = 0, k = 0, wk = 0;
     if (q < nbins) {
       b = pencil_bin(q, nb);
       start = bin_start[b];
       c = bin_start[b + 1] - start;
-      k = (c + (uint32_t)max_sub - 1) / (uint32_t)max_sub;
+      k = (c + ms - 1) / ms;
+      wk = (c + kWorkUnit - 1) / kWorkUnit + k * kSubWork;
     }
-    uint32_t total;
+    uint32_t total, total_w;
     const uint32_t o = carry + block_exclusive_scan<kSubThreads>(k, sh, total);
+    uint32_t w = carry_w + block_exclusive_scan<kSubThreads>(wk, sh, total_w);
     for (uint32_t t = 0; t < k; ++t) {
-      const uint32_t cnt = min(c - t * (uint32_t)max_sub, (uint32_t)max_sub);
-      subs[o + t] = make_int4(b, (int)(start + t * (uint32_t)max_sub), (int)cnt, 0);
+      const uint32_t cnt = min(c - t * ms, ms);
+      subs[o + t] = make_int4(b, (int)(start + t * ms), (int)cnt, (int)w);
+      w += (cnt + kWorkUnit - 1) / kWorkUnit + kSubWork;
     }
     carry += total;
+    carry_w += total_w;
   }
-  if (threadIdx.x == 0) *nsub = carry;
+  if (threadIdx.x == 0) {
+    nsub[0] = carry;
+    nsub[1] = carry_w;
+  }
 }
 
 } // namespace

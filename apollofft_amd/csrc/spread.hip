@@ -763,25 +763,47 @@ __device__ __forceinline__ void mfma_rescale(f32x16 (&acc)[MSpread<W>::XW][2], f
   }
 }
 
-// Contiguous run of the subproblem list for this workgroup: each XCD takes
-// one contiguous eighth (blocks are dealt to the XCDs round-robin, block b
-// on XCD b mod 8) and splits it evenly among its blocks, so a workgroup
-// walks consecutive tiles of a pencil.
+// Work range [w0, w1) of this workgroup out of total work wt: each XCD
+// takes one contiguous eighth (blocks are dealt to the XCDs round-robin,
+// block b on XCD b mod 8) and splits it evenly among its blocks, so a
+// workgroup walks consecutive tiles of a pencil and the XCD's tiles meet in
+// its L2.
+__device__ __forceinline__ void xcd_work_range(uint32_t wt, uint32_t &w0, uint32_t &w1) {
+  constexpr uint32_t kXcd = 8;
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  if (g < kXcd) {
+    w0 = (uint32_t)((uint64_t)wt * b / g);
+    w1 = (uint32_t)((uint64_t)wt * (b + 1) / g);
+    return;
+  }
+  const uint32_t x = b % kXcd, j = b / kXcd;
+  const uint32_t per = (g - x + kXcd - 1) / kXcd;  // blocks on XCD x
+  const uint64_t lo = (uint64_t)wt * x / kXcd, hi = (uint64_t)wt * (x + 1) / kXcd;
+  w0 = (uint32_t)(lo + (hi - lo) * j / per);
+  w1 = (uint32_t)(lo + (hi - lo) * (j + 1) / per);
+}
+
+// First subproblem i in [0, n) whose work prefix subs[i].w is >= w (n if none).
+__device__ __forceinline__ uint32_t work_lower_bound(const int4 *subs, uint32_t n, uint32_t w) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((uint32_t)subs[mid].w < w) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Contiguous run of the subproblem list for this workgroup, balanced by the
+// work prefix of the list (points plus a fixed cost per subproblem).
 struct XcdChunk {
   uint32_t begin, end;
-  __device__ __forceinline__ XcdChunk(uint32_t n) {
-    constexpr uint32_t kXcd = 8;
-    const uint32_t g = gridDim.x, b = blockIdx.x;
-    if (g < kXcd) {
-      begin = (uint32_t)((uint64_t)n * b / g);
-      end = (uint32_t)((uint64_t)n * (b + 1) / g);
-      return;
-    }
-    const uint32_t x = b % kXcd, j = b / kXcd;
-    const uint32_t per = (g - x + kXcd - 1) / kXcd;  // blocks on XCD x
-    const uint64_t lo = (uint64_t)n * x / kXcd, hi = (uint64_t)n * (x + 1) / kXcd;
-    begin = (uint32_t)(lo + (hi - lo) * j / per);
-    end = (uint32_t)(lo + (hi - lo) * (j + 1) / per);
+  __device__ __forceinline__ XcdChunk(const PointArgs &pa) {
+    const uint32_t n = pa.nsub[0];
+    uint32_t w0, w1;
+    xcd_work_range(pa.nsub[1], w0, w1);
+    begin = work_lower_bound(pa.subs, n, w0);
+    end = work_lower_bound(pa.subs, n, w1);
   }
 };
 
@@ -798,7 +820,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
     for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
   stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
   MWindow win;
-  const XcdChunk r(*pa.nsub);
+  const XcdChunk r(pa);
   for (uint32_t sidx = r.begin; sidx < r.end; ++sidx) {
     const int4 sub = pa.subs[sidx];
     const MTile t = mfma_tile<W>(g, sub.x);
@@ -835,7 +857,24 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kerne
     for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
   stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
   MWindow win;
-  const XcdChunk r(prefix[nprob]);
+  // global work range -> global subproblem range (wpre = work prefix of
+  // the problems, after the subproblem prefix)
+  const uint32_t *wpre = prefix + nprob + 1;
+  uint32_t w0, w1;
+  xcd_work_range(wpre[nprob], w0, w1);
+  auto global_lb = [&](uint32_t w) -> uint32_t {
+    if (w >= wpre[nprob]) return prefix[nprob];
+    int lo = 0, hi = nprob - 1;  // last problem with wpre <= w
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (wpre[mid] <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    return prefix[lo] + work_lower_bound(probs[lo].pa.subs, prefix[lo + 1] - prefix[lo], w - wpre[lo]);
+  };
+  struct {
+    uint32_t begin, end;
+  } r{global_lb(w0), global_lb(w1)};
   for (uint32_t gidx = r.begin; gidx < r.end; ++gidx) {
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
@@ -892,15 +931,19 @@ __global__ void batch_table_kernel(BatchChunk chunk, SpreadProblem *table) {
   if (i < chunk.count) table[chunk.first + i] = chunk.probs[i];
 }
 
-// prefix[p] = sum of the subproblem counts of problems < p.
+// prefix[p] = sum of the subproblem counts of problems < p, and
+// prefix[nprob + 1 + p] = sum of their work.
 __global__ void batch_prefix_kernel(const SpreadProblem *probs, int nprob, uint32_t *prefix) {
   if (threadIdx.x != 0) return;
-  uint32_t acc = 0;
+  uint32_t acc = 0, accw = 0;
   for (int p = 0; p < nprob; ++p) {
     prefix[p] = acc;
-    acc += *probs[p].pa.nsub;
+    prefix[nprob + 1 + p] = accw;
+    acc += probs[p].pa.nsub[0];
+    accw += probs[p].pa.nsub[1];
   }
   prefix[nprob] = acc;
+  prefix[2 * nprob + 1] = accw;
 }
 
 // Load the padded tile of subproblem origin (ox,oy,oz) from the grid.
