@@ -807,22 +807,24 @@ struct XcdChunk {
   }
 };
 
+// Spread the subproblems [begin, end) of one problem's list with one ring
+// window: a tile further along the pencil keeps the columns both cover;
+// everything left is flushed at the end.
 template <int W>
-__global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
-    PointArgs pa, ZRec zr, float2 *grid, Geom g, const KbPoly *__restrict__ dpoly) {
+__device__ __forceinline__ void spread_run_mfma(const int4 *__restrict__ subs, uint32_t begin,
+                                                uint32_t end, const ZRec &zr, float2 *grid,
+                                                const Geom &g, const KbPoly *__restrict__ dpoly,
+                                                char *shm) {
   using MS = MSpread<W>;
-  extern __shared__ __align__(16) char shm[];
   float *red = (float *)(shm + MLds<MS::PX>::RED);
   f32x16 acc[MS::XW][2];
 #pragma unroll
   for (int q = 0; q < MS::XW; ++q)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
-  stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
   MWindow win;
-  const XcdChunk r(pa);
-  for (uint32_t sidx = r.begin; sidx < r.end; ++sidx) {
-    const int4 sub = pa.subs[sidx];
+  for (uint32_t sidx = begin; sidx < end; ++sidx) {
+    const int4 sub = subs[sidx];
     const MTile t = mfma_tile<W>(g, sub.x);
     const int e = mfma_strength_exp<MS::NW>(zr, sub, red);
     if (win.prob >= 0) {
@@ -839,67 +841,53 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
     mfma_flush<W>(acc, grid, g, win.t, MS::PX, ldexpf(1.0f, win.e) * kProdUnscale);
 }
 
+template <int W>
+__global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
+    PointArgs pa, ZRec zr, float2 *grid, Geom g, const KbPoly *__restrict__ dpoly) {
+  using MS = MSpread<W>;
+  extern __shared__ __align__(16) char shm[];
+  stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
+  const XcdChunk r(pa);
+  spread_run_mfma<W>(pa.subs, r.begin, r.end, zr, grid, g, dpoly, shm);
+}
+
 // Batched form: one persistent launch over the subproblems of all problems
 // (prefix[p] = first global subproblem index of problem p, prefix[nprob] =
-// total); the window moves on within a problem as above and is flushed
-// whole when the problem changes.
+// total, then the work prefix of the problems).  A workgroup's work range
+// maps to a run of global subproblems, which it spreads problem by problem.
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kernel(
     const SpreadProblem *probs, const uint32_t *prefix, int nprob,
     const KbPoly *__restrict__ dpoly) {
   using MS = MSpread<W>;
   extern __shared__ __align__(16) char shm[];
-  float *red = (float *)(shm + MLds<MS::PX>::RED);
-  f32x16 acc[MS::XW][2];
-#pragma unroll
-  for (int q = 0; q < MS::XW; ++q)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
   stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
-  MWindow win;
-  // global work range -> global subproblem range (wpre = work prefix of
-  // the problems, after the subproblem prefix)
   const uint32_t *wpre = prefix + nprob + 1;
   uint32_t w0, w1;
   xcd_work_range(wpre[nprob], w0, w1);
-  auto global_lb = [&](uint32_t w) -> uint32_t {
-    if (w >= wpre[nprob]) return prefix[nprob];
-    int lo = 0, hi = nprob - 1;  // last problem with wpre <= w
+  // last problem with wpre <= w
+  auto problem_of = [&](uint32_t w) {
+    int lo = 0, hi = nprob - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
       if (wpre[mid] <= w) lo = mid;
       else hi = mid - 1;
     }
-    return prefix[lo] + work_lower_bound(probs[lo].pa.subs, prefix[lo + 1] - prefix[lo], w - wpre[lo]);
+    return lo;
   };
-  struct {
-    uint32_t begin, end;
-  } r{global_lb(w0), global_lb(w1)};
-  for (uint32_t gidx = r.begin; gidx < r.end; ++gidx) {
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (prefix[mid] <= gidx) lo = mid;
-      else hi = mid - 1;
-    }
-    const SpreadProblem &pr = probs[lo];
-    const int4 sub = pr.pa.subs[gidx - prefix[lo]];
-    const MTile t = mfma_tile<W>(pr.g, sub.x);
-    const int e = mfma_strength_exp<MS::NW>(pr.zr, sub, red);
-    if (win.prob >= 0) {
-      const SpreadProblem &pw = probs[win.prob];
-      const int ncol = mfma_flush_cols<W>(win, lo, t, e, pw.g);
-      mfma_flush<W>(acc, pw.grid, pw.g, win.t, ncol, ldexpf(1.0f, win.e) * kProdUnscale);
-      if (ncol < MS::PX && e != win.e) mfma_rescale<W>(acc, ldexpf(1.0f, win.e - e));
-    }
-    win.prob = lo;
-    win.t = t;
-    win.e = e;
-    mfma_accumulate<W>(acc, pr.zr, dpoly, sub, t.xlo & (MS::PX - 1), ldexpf(1.0f, -e), shm);
-  }
-  if (win.prob >= 0) {
-    const SpreadProblem &pw = probs[win.prob];
-    mfma_flush<W>(acc, pw.grid, pw.g, win.t, MS::PX, ldexpf(1.0f, win.e) * kProdUnscale);
+  auto global_lb = [&](uint32_t w) -> uint32_t {
+    if (w >= wpre[nprob]) return prefix[nprob];
+    const int p = problem_of(w);
+    return prefix[p] + work_lower_bound(probs[p].pa.subs, prefix[p + 1] - prefix[p], w - wpre[p]);
+  };
+  const uint32_t gbegin = global_lb(w0), gend = global_lb(w1);
+  if (gbegin >= gend) return;
+  for (int p = problem_of(w0); p < nprob && prefix[p] < gend; ++p) {
+    const uint32_t lb = max(gbegin, prefix[p]) - prefix[p];
+    const uint32_t le = min(gend, prefix[p + 1]) - prefix[p];
+    if (lb >= le) continue;
+    const SpreadProblem &pr = probs[p];
+    spread_run_mfma<W>(pr.pa.subs, lb, le, pr.zr, pr.grid, pr.g, dpoly, shm);
   }
 }
 
