@@ -54,7 +54,7 @@ struct Geom {
 // per kWorkUnit points plus kSubWork units of fixed cost (staging, scale,
 // flush).
 constexpr uint32_t kWorkUnit = 16;
-constexpr uint32_t kSubWork = 16;
+constexpr uint32_t kSubWork = 48;
 
 struct SortBuffers {
   uint32_t *keys[2];
