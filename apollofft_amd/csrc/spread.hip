@@ -37,30 +37,6 @@ constexpr int kSpreadStage = 256;  // points staged per round (spread)
 constexpr int kSpreadThreads = 1024;
 constexpr int kInterpThreads = 1024;
 
-// XCD-aware split of the subproblem list.  Workgroups are dealt to the 8
-// XCDs of the chip round-robin (block b runs on XCD b mod 8); each XCD takes
-// one contiguous eighth of the list (neighbouring tiles, whose atomics and
-// halo reads then meet in that XCD's L2) and its blocks stride through it.
-struct XcdRange {
-  uint32_t begin, end, step;
-  __device__ __forceinline__ XcdRange(uint32_t n) {
-    constexpr uint32_t kXcd = 8;
-    const uint32_t g = gridDim.x, b = blockIdx.x;
-    if (g < kXcd) {
-      begin = b;
-      end = n;
-      step = g;
-      return;
-    }
-    const uint32_t x = b % kXcd, j = b / kXcd;
-    const uint32_t per = (g - x + kXcd - 1) / kXcd;  // blocks on XCD x
-    const uint64_t lo = (uint64_t)n * x / kXcd, hi = (uint64_t)n * (x + 1) / kXcd;
-    begin = (uint32_t)lo + j;
-    end = (uint32_t)hi;
-    step = per;
-  }
-};
-
 // Periodic index; padded tiles may exceed the grid by more than one period
 // when the grid is not much larger than the kernel support.
 __device__ __forceinline__ int wrap(int v, int m) {
@@ -1051,8 +1027,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-  const XcdRange xr(*pa.nsub);
-  for (uint32_t sidx = xr.begin; sidx < xr.end; sidx += xr.step) {
+  const XcdChunk xr(pa);
+  for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
@@ -1264,8 +1240,8 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-  const XcdRange xr(*pa.nsub);
-  for (uint32_t sidx = xr.begin; sidx < xr.end; sidx += xr.step) {
+  const XcdChunk xr(pa);
+  for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
