@@ -708,6 +708,9 @@ __device__ __forceinline__ void mfma_flush(f32x16 (&acc)[MSpread<W>::XW][2], flo
   }
 }
 
+// Longest run of points accumulated in registers between full flushes.
+constexpr int kMaxChain = 4096;
+
 // Ring window of one workgroup: the open tile, its problem and scale.
 struct MWindow {
   int prob = -1;  // problem of the open tile, -1 = none
@@ -799,15 +802,21 @@ __device__ __forceinline__ void spread_run_mfma(const int4 *__restrict__ subs, u
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[q][0][r] = acc[q][1][r] = 0.f;
   MWindow win;
+  int chain = 0;  // points accumulated since the last full flush
   for (uint32_t sidx = begin; sidx < end; ++sidx) {
     const int4 sub = subs[sidx];
     const MTile t = mfma_tile<W>(g, sub.x);
     const int e = mfma_strength_exp<MS::NW>(zr, sub, red);
     if (win.prob >= 0) {
-      const int ncol = mfma_flush_cols<W>(win, 0, t, e, g);
+      int ncol = mfma_flush_cols<W>(win, 0, t, e, g);
+      // bound the FP32 accumulation chains (rounding error grows with their
+      // length) as a full flush per subproblem of kMaxChain points would
+      if (chain + sub.z > kMaxChain) ncol = MS::PX;
       mfma_flush<W>(acc, grid, g, win.t, ncol, ldexpf(1.0f, win.e) * kProdUnscale);
       if (ncol < MS::PX && e != win.e) mfma_rescale<W>(acc, ldexpf(1.0f, win.e - e));
+      if (ncol == MS::PX) chain = 0;
     }
+    chain += sub.z;
     win.prob = 0;
     win.t = t;
     win.e = e;
