@@ -709,7 +709,7 @@ __device__ __forceinline__ void mfma_flush(f32x16 (&acc)[MSpread<W>::XW][2], flo
 }
 
 // Longest run of points accumulated in registers between full flushes.
-constexpr int kMaxChain = 4096;
+constexpr int kMaxChain = 8192;
 
 // Ring window of one workgroup: the open tile, its problem and scale.
 struct MWindow {
