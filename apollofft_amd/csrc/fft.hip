@@ -189,9 +189,17 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   float2 *buf0 = sh;
   float2 *buf1 = sh + (size_t)lb * Lf;
   float2 *tw = sh + 2 * (size_t)lb * Lf;  // twiddle table of length Lf
+  int64_t *lbase = (int64_t *)(tw + Lf);  // [lb] input, [lb] output line offsets
   for (int i = threadIdx.x; i < Lf; i += kFftThreads) tw[i] = p.tw[i];
   const int64_t l0 = (int64_t)blockIdx.x * lb;
   const int nl = (int)((p.nlines - l0) < lb ? (p.nlines - l0) : lb);
+  // element offsets of the block's lines, one 64-bit division per line
+  for (int li = threadIdx.x; li < nl; li += kFftThreads) {
+    const int64_t l = l0 + li, a = l / p.nb, b = l - a * p.nb;
+    lbase[li] = a * p.in_sa + b * p.in_sb;
+    lbase[lb + li] = a * p.out_sa + b * p.out_sb;
+  }
+  __syncthreads();
 
   if (blue) {
     // a_n = x_n chirp_n on [0, L), zero on [L, Lf): clear everything first
@@ -209,9 +217,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
       i = idx / nl;
       li = idx - i * nl;
     }
-    int64_t l = l0 + li;
-    int64_t a = l / p.nb, b = l - a * p.nb;
-    float2 v = p.in[a * p.in_sa + b * p.in_sb + (int64_t)i * p.in_si];
+    float2 v = p.in[lbase[li] + (int64_t)i * p.in_si];
     if (p.dec_in) {
       float f = (float)p.dec_in[i];
       v.x *= f;
@@ -255,8 +261,6 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
       o = idx / nl;
       li = idx - o * nl;
     }
-    int64_t l = l0 + li;
-    int64_t a = l / p.nb, b = l - a * p.nb;
     const int pos = kept_pos(o, p.nout, L);
     float2 v = src[li * Lf + pos];
     if (blue) v = cmul(make_float2(v.x, -v.y), p.chirp[pos]);  // X_k = chirp_k conj(c_k)
@@ -264,7 +268,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     float f = (float)(p.dec_out ? p.scale * p.dec_out[o] : p.scale);
     v.x *= f;
     v.y *= f;
-    p.out[a * p.out_sa + b * p.out_sb + (int64_t)o * p.out_si] = v;
+    p.out[lbase[lb + li] + (int64_t)o * p.out_si] = v;
   }
 }
 
@@ -280,7 +284,7 @@ hipError_t launch_fft_pass(const FftPass &p, hipStream_t s) {
   if (p.nlines <= 0) return hipSuccess;
   int64_t blocks = (p.nlines + p.lb - 1) / p.lb;
   const int Lf = p.blue_n > 0 ? p.blue_n : p.L;
-  size_t lds = (2 * (size_t)p.lb + 1) * Lf * sizeof(float2);
+  size_t lds = (2 * (size_t)p.lb + 1) * Lf * sizeof(float2) + 2 * (size_t)p.lb * sizeof(int64_t);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)fft_pass_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
