@@ -39,7 +39,8 @@ EXPORTS = [
     "anufft_workspace_size", "anufft_execute_type1", "anufft_execute_type2",
     "anufft_execute_grad_points", "anufft_batch_workspace_size",
     "anufft_execute_type1_batched", "anufft_execute_type1_host",
-    "anufft_execute_type2_host", "anufft_last_error", "anufft_version",
+    "anufft_execute_type2_host", "anufft_direct_type1", "anufft_direct_type2",
+    "anufft_last_error", "anufft_version",
     "anufft_debug_num_bins", "anufft_debug_sort", "anufft_debug_spread",
     "anufft_timing_enable", "anufft_timing_collect",
 ]
@@ -163,6 +164,8 @@ def lib():
                                                vp, sz, vp]),
         "anufft_execute_type1_host": (i32, [vp, i32, i32, i64, P(vp), i64, vp, vp]),
         "anufft_execute_type2_host": (i32, [vp, i32, i32, i64, P(vp), i64, vp, vp]),
+        "anufft_direct_type1": (i32, [vp, i64, P(vp), i64, vp, vp, vp]),
+        "anufft_direct_type2": (i32, [vp, i64, P(vp), i64, vp, vp, vp]),
         "anufft_debug_num_bins": (i32, [vp, P(i64)]),
         "anufft_debug_sort": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
         "anufft_debug_spread": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),

@@ -179,6 +179,10 @@ hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const 
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
                          const KbPoly &poly, const KbPoly *dpoly, int num_blocks, hipStream_t s);
 
+// Direct FP64 sums (direct.hip): type 1 writes every mode, type 2 every point.
+hipError_t launch_direct(int type, const PointArgs &pa, int dim, const int64_t n[3],
+                         const double L[3], const float2 *in, double2 *out, hipStream_t s);
+
 hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t num_points,
                            const float2 *values, float2 *grid, const Geom &g,
                            const KbPoly &poly, hipStream_t s);

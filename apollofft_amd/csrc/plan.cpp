@@ -770,6 +770,32 @@ anufft_status check_exec(const anufft_plan *p, int64_t M, const float *const coo
   return ANUFFT_OK;
 }
 
+// Exact direct sums (anufft_direct_type1 / _type2).
+anufft_status direct(int type, const anufft_plan *p, int64_t M, const float *const coords[3],
+                     int64_t stride, const float *in, double *out, void *stream) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  if (M < 0) return fail(ANUFFT_INVALID_ARG, "num_points must be >= 0");
+  if (M > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points %lld exceeds u32", (long long)M);
+  if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
+    return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
+  if (stride < 1) return fail(ANUFFT_INVALID_ARG, "coordinate stride must be >= 1");
+  if (M > 0 && !in) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
+  if (!out && (type == 1 || M > 0)) return fail(ANUFFT_INVALID_ARG, "output pointer is NULL");
+  if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  if (type == 2 && M == 0) return ANUFFT_OK;
+  DeviceGuard guard(p->device);
+  PointArgs pa{};
+  pa.x = M > 0 ? coords[0] : nullptr;
+  pa.y = M > 0 && p->dim == 3 ? coords[1] : pa.x;
+  pa.z = M > 0 && p->dim == 3 ? coords[2] : pa.x;
+  pa.stride = stride;
+  pa.num_points = M;
+  hipError_t e = launch_direct(type, pa, p->dim, p->n, p->L, (const float2 *)in, (double2 *)out,
+                               (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "direct sum");
+  return ANUFFT_OK;
+}
+
 } // namespace
 
 // ============================================================================
@@ -1041,6 +1067,16 @@ anufft_status anufft_timing_collect(const anufft_plan *p, double *ms, int64_t *c
   p->timer.recs.clear();
   if (err != hipSuccess) return hip_fail(err, "timing_collect");
   return ANUFFT_OK;
+}
+
+anufft_status anufft_direct_type1(const anufft_plan *p, int64_t M, const float *const coords[3],
+                                  int64_t stride, const float *values, double *out, void *stream) {
+  return direct(1, p, M, coords, stride, values, out, stream);
+}
+
+anufft_status anufft_direct_type2(const anufft_plan *p, int64_t M, const float *const coords[3],
+                                  int64_t stride, const float *modes, double *out, void *stream) {
+  return direct(2, p, M, coords, stride, modes, out, stream);
 }
 
 anufft_status anufft_debug_num_bins(const anufft_plan *p, int64_t *num_bins) {

@@ -185,6 +185,33 @@ class _PlanBase:
             ws.data_ptr(), ws.numel(), self._stream(dev)), "execute_type2")
         return out
 
+    def direct_type1(self, positions, values):
+        """Exact Type-1 sum in FP64 on the device -> complex128 modes (the
+        WGPU direct backend, nufft.wgsl:92-117, computed there in f32)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        if values.dtype != torch.complex64 or not values.is_contiguous() or values.numel() != M:
+            raise ValueError("values must be a contiguous complex64 tensor with one entry per point")
+        out = torch.empty(self.modes_shape, dtype=torch.complex128, device=dev)
+        N.check(N.lib().anufft_direct_type1(self._h, M, N.ptr_array(ptrs), stride,
+                                            values.data_ptr(), out.data_ptr(), self._stream(dev)),
+                "direct_type1")
+        return out
+
+    def direct_type2(self, positions, modes):
+        """Exact Type-2 sum in FP64 on the device -> complex128 per point
+        (the WGPU direct backend, nufft.wgsl:119-140)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        shape = self.modes_shape
+        if modes.dtype != torch.complex64 or not modes.is_contiguous() or tuple(modes.shape) != shape:
+            raise ValueError(f"modes must be a contiguous complex64 tensor of shape {shape}")
+        out = torch.empty(M, dtype=torch.complex128, device=dev)
+        N.check(N.lib().anufft_direct_type2(self._h, M, N.ptr_array(ptrs), stride,
+                                            modes.data_ptr(), out.data_ptr(), self._stream(dev)),
+                "direct_type2")
+        return out
+
     def grad_points(self, positions, u, modes_h):
         """grad[a][j] = Re(conj(u_j) d/dx_j^(a) sum_k h_k e^{+i k.x_j}) -> (M, dim)."""
         import torch

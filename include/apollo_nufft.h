@@ -207,6 +207,26 @@ anufft_status anufft_execute_type2_host(const anufft_plan *plan, int32_t storage
                                         const double *const coords[3], int64_t coord_stride,
                                         const void *modes, void *out);
 
+/* Exact direct sums on the device (replaces the WGPU direct kernels
+ * nufft_type{1,2}_{1d,3d}, apollo-nufft-wgpu infrastructure/shaders/
+ * nufft.wgsl:51-140, and their drivers NufftWgpuKernel::execute_type1_1d,
+ * execute_type2_1d, execute_type1_3d, execute_type2_3d, kernel.rs:652-812,
+ * reached through device.rs:112-216).  Over the plan's domain (n, L = n d):
+ *   type1: out[k] = sum_j c_j exp(-2 pi i sum_a k_a x_{j,a} / L_a), all modes
+ *          in FFT order ([nx][ny][nz], k_a signed: index <= n/2 -> index,
+ *          else index - n);
+ *   type2: out[j] = sum_k f_k exp(+2 pi i sum_a k_a x_{j,a} / L_a).
+ * Device pointers: float32 coordinates (stride as for execute), complex64
+ * values / modes; out is complex128 (interleaved doubles).  The reference
+ * computes these sums in f32 and returns Complex64; here the arithmetic is
+ * FP64.  No workspace; O(num_points * modes) work, queued on `stream`. */
+anufft_status anufft_direct_type1(const anufft_plan *plan, int64_t num_points,
+                                  const float *const coords[3], int64_t coord_stride,
+                                  const float *values, double *out, void *stream);
+anufft_status anufft_direct_type2(const anufft_plan *plan, int64_t num_points,
+                                  const float *const coords[3], int64_t coord_stride,
+                                  const float *modes, double *out, void *stream);
+
 /* Per-stage timing with HIP events recorded on the execution stream
  * (no reference counterpart; the reference records no timings, every WGPU
  * pass sets timestamp_writes: None, kernel.rs:876).  While enabled, each

@@ -172,6 +172,54 @@ def test_three_digit_binning_with_records(dev, oracle_mod):
     assert rel_l2(out2, ref2) < TOL
 
 
+# ---------------------------------------------------------------- direct sums
+@pytest.mark.parametrize("layout", ["aos", "soa"])
+def test_direct_sums_3d_vs_oracle(dev, oracle_mod, layout):
+    """FP64 device direct sums (the WGPU direct backend's sums) against the
+    oracle's FP64 direct sums on the same float32 coordinates."""
+    rng = np.random.default_rng(41)
+    n = (12, 10, 16)
+    plan, d = _plan3(n, w=4)
+    M = 300
+    xyz = uniform_points(rng, M) * np.float32(1.7)  # some points outside one period
+    c = cnormal(rng, M)
+    f = cnormal(rng, n)
+    pos = _t(xyz, dev) if layout == "aos" else tuple(_t(xyz[:, a].copy(), dev) for a in range(3))
+    out1 = plan.direct_type1(pos, _t(c, dev)).cpu().numpy()
+    ref1 = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d)
+    assert out1.dtype == np.complex128 and out1.shape == n
+    assert rel_l2(out1.ravel(), ref1.ravel()) < 1e-12
+    out2 = plan.direct_type2(pos, _t(f, dev)).cpu().numpy()
+    ref2 = oracle_mod.direct_type2_3d(f.astype(np.complex128), xyz.astype(np.float64), d)
+    assert rel_l2(out2, ref2) < 1e-12
+
+
+def test_direct_sums_1d_long_lines_and_empty(dev, oracle_mod):
+    """1D direct sums with n = 2048: the Type-2 mode line spans two LDS
+    chunks of 1024 and its negative-index run starts inside the second; no
+    points: Type-1 gives zeros, Type-2 nothing."""
+    import apollofft_amd as A
+    rng = np.random.default_rng(42)
+    n, M = 2048, 200
+    dx = TWO_PI / n
+    plan = A.NufftPlan1D(n, dx, 2, 6)
+    x = rng.uniform(-np.pi, np.pi, M).astype(np.float32)
+    c = cnormal(rng, M)
+    f = cnormal(rng, n)
+    out1 = plan.direct_type1(_t(x, dev), _t(c, dev)).cpu().numpy()
+    ref1 = oracle_mod.direct_type1_1d(x.astype(np.float64), c.astype(np.complex128), n, dx)
+    assert rel_l2(out1, ref1) < 1e-12
+    out2 = plan.direct_type2(_t(x, dev), _t(f, dev)).cpu().numpy()
+    ref2 = oracle_mod.direct_type2_1d(f.astype(np.complex128), x.astype(np.float64), dx)
+    assert rel_l2(out2, ref2) < 1e-12
+    import torch
+    z = plan.direct_type1(torch.empty(0, dtype=torch.float32, device=dev),
+                          torch.empty(0, dtype=torch.complex64, device=dev)).cpu().numpy()
+    assert z.shape == (n,) and not np.any(z)
+    e = plan.direct_type2(torch.empty(0, dtype=torch.float32, device=dev), _t(f, dev))
+    assert e.numel() == 0
+
+
 def test_type2_3d_compat_scaling(dev, oracle_mod):
     """compat flag reproduces the reference's direct/(mx my mz) output
     (dimension_3d.rs:549-551)."""
