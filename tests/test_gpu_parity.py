@@ -220,6 +220,21 @@ def test_direct_sums_1d_long_lines_and_empty(dev, oracle_mod):
     assert e.numel() == 0
 
 
+def test_validation_report_passes(dev):
+    """The reference's five NUFFT suite cases (suite.rs:227-309) in the
+    NufftReport schema (report.rs:60-73); the reference's report test asserts
+    report.nufft.passed (suite.rs:2694)."""
+    from apollofft_amd.validation import NUFFT_FAST_RELATIVE_LIMIT, run_nufft_suite
+    r = run_nufft_suite(dev.index or 0)
+    assert set(r) == {"type1_1d_max_relative_error", "type2_1d_max_relative_error",
+                      "type1_3d_max_relative_error", "irrational_positions_max_relative_error",
+                      "clustered_positions_max_relative_error", "passed"}
+    for k, v in r.items():
+        if k != "passed":
+            assert 0.0 <= v <= NUFFT_FAST_RELATIVE_LIMIT, (k, v)
+    assert r["passed"] is True
+
+
 def test_type2_3d_compat_scaling(dev, oracle_mod):
     """compat flag reproduces the reference's direct/(mx my mz) output
     (dimension_3d.rs:549-551)."""

@@ -148,3 +148,30 @@ def test_python_api_argument_checks():
         A.nufft_type1_3d_fast(pos, np.zeros(9, complex), 4, 4, 4, 0.1, 0.1, 0.1)
     with pytest.raises(ValueError, match="mismatch"):
         A.nufft_type1_1d_fast(np.zeros(4), np.zeros(3, complex), 0.1)
+
+
+def test_direct_sum_argument_checks():
+    """anufft_direct_type1/2 reject bad arguments before touching a device."""
+    lib = N.lib()
+    nul = N.ptr_array([None] * 3)
+    assert lib.anufft_direct_type1(None, 0, nul, 1, None, None, None) == N.INVALID_ARG
+    p = A.NufftPlan3D(4, 4, 4, 0.1, 0.1, 0.1)
+    assert lib.anufft_direct_type1(p.handle, -1, nul, 1, None, None, None) == N.INVALID_ARG
+    assert lib.anufft_direct_type2(p.handle, 3, nul, 1, None, None, None) == N.INVALID_ARG
+    buf = np.zeros(64, np.float32)
+    ptrs = N.ptr_array([buf.ctypes.data] * 3)
+    assert lib.anufft_direct_type1(p.handle, 3, ptrs, 0, buf.ctypes.data, buf.ctypes.data,
+                                   None) == N.INVALID_ARG  # stride 0
+    assert lib.anufft_direct_type2(p.handle, 3, ptrs, 3, None, buf.ctypes.data,
+                                   None) == N.INVALID_ARG  # no modes
+    assert lib.anufft_direct_type1(p.handle, 0, nul, 1, None, None, None) == N.INVALID_ARG  # no out
+
+
+def test_validation_metric_is_the_reference_one():
+    """relative_complex_error = max |e - f| / max(|e|, 1) (suite.rs:2665-2674)."""
+    from apollofft_amd.validation import relative_complex_error
+    e = np.array([2.0, 0.5j, 0.0])
+    f = np.array([2.3, 0.5j + 0.1, 0.0])
+    assert relative_complex_error(e, f) == pytest.approx(0.15)
+    assert relative_complex_error(e, e) == 0.0
+    assert relative_complex_error([], []) == 0.0
