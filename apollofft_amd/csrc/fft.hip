@@ -272,10 +272,17 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   }
 }
 
+// Zero n complex values: 16-byte stores (two values) where the pointer is
+// 16-byte aligned, the odd tail (and an unaligned start) element-wise.
 __global__ void fill_kernel(float2 *p, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += st) p[i] = make_float2(0.f, 0.f);
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t head = ((uintptr_t)p & 15) ? 1 : 0;  // float2 is 8-byte aligned
+  if (i0 == 0 && head && n > 0) p[0] = make_float2(0.f, 0.f);
+  float4 *q = (float4 *)(p + head);
+  const int64_t n2 = (n - head) / 2;
+  for (int64_t i = i0; i < n2; i += st) q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i0 == 0 && n - head - 2 * n2 == 1) p[n - 1] = make_float2(0.f, 0.f);
 }
 
 } // namespace

@@ -84,23 +84,30 @@ __global__ __launch_bounds__(kSortThreads) void hist_kernel(KeySrc keys, int64_t
 }
 
 // ---- exclusive scan of a uint32 array (reduce, scan block sums, apply) ----
-template <int NT = kScanThreads>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh,
-                                                         uint32_t &total) {
-  // Hillis-Steele over the block in LDS; NT entries.
-  int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int off = 1; off < NT; off <<= 1) {
-    uint32_t add = t >= off ? sh[t - off] : 0u;
-    __syncthreads();
-    sh[t] += add;
-    __syncthreads();
+// Exclusive scan over a block of NT threads (NT / 64 waves): wave scans by
+// shuffles, wave totals through LDS (sh: NT / 64 entries); two barriers.
+template <int NT>
+__device__ __forceinline__ uint32_t block_scan_waves(uint32_t v, uint32_t *sh, uint32_t &total) {
+  constexpr int NWV = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
   }
-  uint32_t incl = sh[t];
-  total = sh[NT - 1];
+  if (lane == 63) sh[wave] = x;
   __syncthreads();
-  return incl - v;
+  uint32_t before = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < NWV; ++w) {
+    const uint32_t t = sh[w];
+    before += w < wave ? t : 0u;
+    total += t;
+  }
+  __syncthreads();  // sh may be reused
+  return before + x - v;
 }
 
 __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *data,
@@ -113,7 +120,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_
     if (j < n) s += data[j];
   }
   uint32_t total;
-  block_exclusive_scan(s, sh, total);
+  block_scan_waves<kScanThreads>(s, sh, total);
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
@@ -125,7 +132,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *data
   uint32_t before = 0;
   for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) before += sums[i];
   uint32_t boff;
-  block_exclusive_scan(before, sh, boff);
+  block_scan_waves<kScanThreads>(before, sh, boff);
   int64_t base = (int64_t)blockIdx.x * kScanTile;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -135,7 +142,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *data
     s += v[i];
   }
   uint32_t total;
-  uint32_t ex = block_exclusive_scan(s, sh, total) + boff;
+  uint32_t ex = block_scan_waves<kScanThreads>(s, sh, total) + boff;
   for (int i = 0; i < kScanItems; ++i) {
     int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
     if (j < n) data[j] = ex;
@@ -289,8 +296,8 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t
       wk = (c + kWorkUnit - 1) / kWorkUnit + k * kSubWork;
     }
     uint32_t total, total_w;
-    const uint32_t o = carry + block_exclusive_scan<kSubThreads>(k, sh, total);
-    uint32_t w = carry_w + block_exclusive_scan<kSubThreads>(wk, sh, total_w);
+    const uint32_t o = carry + block_scan_waves<kSubThreads>(k, sh, total);
+    uint32_t w = carry_w + block_scan_waves<kSubThreads>(wk, sh, total_w);
     for (uint32_t t = 0; t < k; ++t) {
       const uint32_t cnt = min(c - t * ms, ms);
       subs[o + t] = make_int4(b, (int)(start + t * ms), (int)cnt, (int)w);
