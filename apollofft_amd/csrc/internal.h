@@ -172,6 +172,10 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
                                    const KbPoly &poly, const KbPoly *dpoly, int num_blocks,
                                    hipStream_t s);
 // MFMA interpolation needs 2 nbins floats of scratch (bin and tile maxima).
+// Tile maxima of the grid for the MFMA interpolation (no-op for the vector
+// ALU method); launch_interp3d reads them from the same scratch.
+hipError_t launch_tile_max3d(int w, int method, const float2 *grid, float *scratch, const Geom &g,
+                             hipStream_t s);
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
                            const KbPoly *dpoly, int num_blocks, hipStream_t s);

@@ -221,6 +221,12 @@ struct anufft_plan {
   int lds_max = 163840; // gfx950 LDS per workgroup
   int kernel_method = kSpreadMfma;
   hipStream_t host_stream = nullptr;
+  // Second stream for the Type-2 FFT passes, which run concurrently with the
+  // binning of the points (fork/join through the two events; the mutex keeps
+  // concurrent callers from interleaving their record/wait pairs).
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  mutable std::mutex fork_mu;
   mutable StageTimer timer;
 
   int64_t modes() const { return n[0] * n[1] * n[2]; }
@@ -384,6 +390,9 @@ anufft_status build_tables(anufft_plan *p) {
     if (e != hipSuccess) return hip_fail(e, "device kernel polynomials");
   }
   hipError_t es = hipStreamCreateWithFlags(&p->host_stream, hipStreamNonBlocking);
+  if (es == hipSuccess) es = hipStreamCreateWithFlags(&p->aux_stream, hipStreamNonBlocking);
+  if (es == hipSuccess) es = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  if (es == hipSuccess) es = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
   if (es != hipSuccess) return hip_fail(es, "hipStreamCreate");
   for (int a = 0; a < p->dim; ++a) {
     hipError_t e = hipMalloc(&p->d_deconv[a], sizeof(double) * p->n[a]);
@@ -422,6 +431,9 @@ void destroy_plan(anufft_plan *p) {
     }
     if (p->d_poly) (void)hipFree(p->d_poly);
     if (p->host_stream) (void)hipStreamDestroy(p->host_stream);
+    if (p->aux_stream) (void)hipStreamDestroy(p->aux_stream);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
     for (auto &r : p->timer.recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -970,12 +982,31 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
   hipError_t e;
   if (p->dim == 3) {
     PointArgs pa = ex.point_args(p->bi, M, coords, stride);
-    e = ex.bin(p->bi, pa, nullptr, p->kernel_method == kSpreadMfma);
-    if (e != hipSuccess) return hip_fail(e, "binning");
-    e = ex.timed(ANUFFT_STAGE_FFT, [&] {
-      return ex.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
-    });
-    if (e != hipSuccess) return hip_fail(e, "fft");
+    // the FFT passes (modes -> grid) and the binning (points) are independent:
+    // the FFT goes to the plan's second stream, joined before interpolation
+    {
+      std::lock_guard<std::mutex> lk(p->fork_mu);
+      Exec ea = ex;
+      ea.s = p->aux_stream;
+      e = hipEventRecord(p->ev_fork, ex.s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(ea.s, p->ev_fork, 0);
+      if (e != hipSuccess) return hip_fail(e, "fork");
+      e = ea.timed(ANUFFT_STAGE_FFT, [&] {
+        return ea.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
+      });
+      if (e != hipSuccess) return hip_fail(e, "fft");
+      e = ea.timed(ANUFFT_STAGE_INTERP, [&] {
+        return launch_tile_max3d(p->w, p->kernel_method, ea.grid(), (float *)(ea.ws + lay.tmax),
+                                 p->bi.g, ea.s);
+      });
+      if (e != hipSuccess) return hip_fail(e, "tile maxima");
+      e = hipEventRecord(p->ev_join, ea.s);
+      if (e != hipSuccess) return hip_fail(e, "join");
+      e = ex.bin(p->bi, pa, nullptr, p->kernel_method == kSpreadMfma);
+      if (e != hipSuccess) return hip_fail(e, "binning");
+      e = hipStreamWaitEvent(ex.s, p->ev_join, 0);
+      if (e != hipSuccess) return hip_fail(e, "join");
+    }
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
       return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(),
                              (float *)(ex.ws + lay.tmax), (float2 *)out, p->bi.g, p->poly, p->d_poly,

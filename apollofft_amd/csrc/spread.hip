@@ -1783,11 +1783,10 @@ struct Spread3Batched {
 };
 
 template <int W>
-struct Interp3Mfma {
-  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
-                        float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
-    // bin maxima -> (x) -> (y) -> (z): tile maxima in tmax, ping-ponging
-    // between the two halves of the scratch
+struct TileMax3 {
+  static hipError_t run(const float2 *grid, float *bmax, const Geom &g, hipStream_t s) {
+    // bin maxima -> (x) -> (y) -> (z): tile maxima in the second half of
+    // the scratch, ping-ponging between its two halves
     const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
     float *tmax = bmax + nbins;
     const unsigned tb = (unsigned)((nbins + 63) / 64);
@@ -1798,6 +1797,15 @@ struct Interp3Mfma {
                        -W, g.pad[1], bmax);
     hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)bmax, g, 2,
                        -W, g.pad[2], tmax);
+    return hipGetLastError();
+  }
+};
+
+template <int W>
+struct Interp3Mfma {
+  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
+                        float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
+    const float *tmax = bmax + g.nbin[0] * g.nbin[1] * g.nbin[2];  // from TileMax3
     constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -1950,6 +1958,11 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
                                     nprob, poly, nblk, s);
 }
 
+hipError_t launch_tile_max3d(int w, int method, const float2 *grid, float *scratch, const Geom &g,
+                             hipStream_t s) {
+  if (method != kSpreadMfma) return hipSuccess;
+  return dispatch_w<TileMax3>(w, grid, scratch, g, s);
+}
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
                            const KbPoly *dpoly, int nblk, hipStream_t s) {
