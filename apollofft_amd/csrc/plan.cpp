@@ -949,20 +949,37 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   if (st != ANUFFT_OK) return st;
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
-  hipError_t e = ex.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ex.grid(), p->cells(), ex.s); });
-  if (e != hipSuccess) return hip_fail(e, "fill");
-  if (M > 0) {
-    if (p->dim == 3) {
-      PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+  hipError_t e = hipSuccess;
+  if (M > 0 && p->dim == 3) {
+    PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+    {
+      // zeroing the grid (second stream) is independent of the binning
+      std::lock_guard<std::mutex> lk(p->fork_mu);
+      Exec ea = ex;
+      ea.s = p->aux_stream;
+      e = hipEventRecord(p->ev_fork, ex.s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(ea.s, p->ev_fork, 0);
+      if (e != hipSuccess) return hip_fail(e, "fork");
+      e = ea.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ea.grid(), p->cells(), ea.s); });
+      if (e != hipSuccess) return hip_fail(e, "fill");
+      e = hipEventRecord(p->ev_join, ea.s);
+      if (e != hipSuccess) return hip_fail(e, "join");
       e = ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma);
       if (e != hipSuccess) return hip_fail(e, "binning");
-      e = ex.spread3(pa, (const float2 *)values, ex.grid());
-    } else {
-      e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
-        return launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(),
-                               p->bs.g, p->poly, ex.s);
-      });
+      e = hipStreamWaitEvent(ex.s, p->ev_join, 0);
+      if (e != hipSuccess) return hip_fail(e, "join");
     }
+    e = ex.spread3(pa, (const float2 *)values, ex.grid());
+    if (e != hipSuccess) return hip_fail(e, "spread");
+  } else {
+    e = ex.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ex.grid(), p->cells(), ex.s); });
+    if (e != hipSuccess) return hip_fail(e, "fill");
+  }
+  if (M > 0 && p->dim != 3) {
+    e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
+      return launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(),
+                             p->bs.g, p->poly, ex.s);
+    });
     if (e != hipSuccess) return hip_fail(e, "spread");
   }
   e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.grid_to_modes((float2 *)out); });
