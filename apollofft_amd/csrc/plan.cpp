@@ -643,6 +643,23 @@ struct Exec {
     if (a && b) t.recs.push_back({stage, a, b});
     return e;
   }
+  // Run aux (an Exec on the plan's second stream) concurrently with main
+  // (on this stream); this stream then waits for aux.  The record/wait pairs
+  // are enqueued under the plan's mutex.
+  template <typename A, typename B>
+  hipError_t fork_join(A aux, B main) const {
+    std::lock_guard<std::mutex> lk(p->fork_mu);
+    Exec ea = *this;
+    ea.s = p->aux_stream;
+    hipError_t e = hipEventRecord(p->ev_fork, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(ea.s, p->ev_fork, 0);
+    if (e == hipSuccess) e = aux(ea);
+    if (e == hipSuccess) e = hipEventRecord(p->ev_join, ea.s);
+    if (e != hipSuccess) return e;
+    hipError_t em = main();
+    hipError_t ej = hipStreamWaitEvent(s, p->ev_join, 0);
+    return em != hipSuccess ? em : ej;
+  }
   float2 *grid() const { return (float2 *)(ws + lay.grid); }
   float *rec() const { return (float *)(ws + lay.rec); }
   uint8_t *xo() const { return (uint8_t *)(ws + lay.xo); }
@@ -952,23 +969,13 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   hipError_t e = hipSuccess;
   if (M > 0 && p->dim == 3) {
     PointArgs pa = ex.point_args(p->bs, M, coords, stride);
-    {
-      // zeroing the grid (second stream) is independent of the binning
-      std::lock_guard<std::mutex> lk(p->fork_mu);
-      Exec ea = ex;
-      ea.s = p->aux_stream;
-      e = hipEventRecord(p->ev_fork, ex.s);
-      if (e == hipSuccess) e = hipStreamWaitEvent(ea.s, p->ev_fork, 0);
-      if (e != hipSuccess) return hip_fail(e, "fork");
-      e = ea.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ea.grid(), p->cells(), ea.s); });
-      if (e != hipSuccess) return hip_fail(e, "fill");
-      e = hipEventRecord(p->ev_join, ea.s);
-      if (e != hipSuccess) return hip_fail(e, "join");
-      e = ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma);
-      if (e != hipSuccess) return hip_fail(e, "binning");
-      e = hipStreamWaitEvent(ex.s, p->ev_join, 0);
-      if (e != hipSuccess) return hip_fail(e, "join");
-    }
+    // zeroing the grid (second stream) is independent of the binning
+    e = ex.fork_join(
+        [&](const Exec &ea) {
+          return ea.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ea.grid(), p->cells(), ea.s); });
+        },
+        [&] { return ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma); });
+    if (e != hipSuccess) return hip_fail(e, "fill / binning");
     e = ex.spread3(pa, (const float2 *)values, ex.grid());
     if (e != hipSuccess) return hip_fail(e, "spread");
   } else {
@@ -1001,29 +1008,20 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     PointArgs pa = ex.point_args(p->bi, M, coords, stride);
     // the FFT passes (modes -> grid) and the binning (points) are independent:
     // the FFT goes to the plan's second stream, joined before interpolation
-    {
-      std::lock_guard<std::mutex> lk(p->fork_mu);
-      Exec ea = ex;
-      ea.s = p->aux_stream;
-      e = hipEventRecord(p->ev_fork, ex.s);
-      if (e == hipSuccess) e = hipStreamWaitEvent(ea.s, p->ev_fork, 0);
-      if (e != hipSuccess) return hip_fail(e, "fork");
-      e = ea.timed(ANUFFT_STAGE_FFT, [&] {
-        return ea.modes_to_grid((const float2 *)modes, p->opts.compat_reference_type2_3d_scaling != 0);
-      });
-      if (e != hipSuccess) return hip_fail(e, "fft");
-      e = ea.timed(ANUFFT_STAGE_INTERP, [&] {
-        return launch_tile_max3d(p->w, p->kernel_method, ea.grid(), (float *)(ea.ws + lay.tmax),
-                                 p->bi.g, ea.s);
-      });
-      if (e != hipSuccess) return hip_fail(e, "tile maxima");
-      e = hipEventRecord(p->ev_join, ea.s);
-      if (e != hipSuccess) return hip_fail(e, "join");
-      e = ex.bin(p->bi, pa, nullptr, p->kernel_method == kSpreadMfma);
-      if (e != hipSuccess) return hip_fail(e, "binning");
-      e = hipStreamWaitEvent(ex.s, p->ev_join, 0);
-      if (e != hipSuccess) return hip_fail(e, "join");
-    }
+    e = ex.fork_join(
+        [&](const Exec &ea) {
+          hipError_t r = ea.timed(ANUFFT_STAGE_FFT, [&] {
+            return ea.modes_to_grid((const float2 *)modes,
+                                    p->opts.compat_reference_type2_3d_scaling != 0);
+          });
+          if (r != hipSuccess) return r;
+          return ea.timed(ANUFFT_STAGE_INTERP, [&] {
+            return launch_tile_max3d(p->w, p->kernel_method, ea.grid(),
+                                     (float *)(ea.ws + lay.tmax), p->bi.g, ea.s);
+          });
+        },
+        [&] { return ex.bin(p->bi, pa, nullptr, p->kernel_method == kSpreadMfma); });
+    if (e != hipSuccess) return hip_fail(e, "fft / binning");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
       return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(),
                              (float *)(ex.ws + lay.tmax), (float2 *)out, p->bi.g, p->poly, p->d_poly,
@@ -1059,10 +1057,13 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
     const bool gm = grad_uses_mfma(p->w, p->kernel_method);
     const anufft_plan::Binning &b = gm ? p->bi : p->bg;
     PointArgs pa = ex.point_args(b, M, coords, stride);
-    e = ex.bin(b, pa, nullptr, gm);
-    if (e != hipSuccess) return hip_fail(e, "binning");
-    e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
-    if (e != hipSuccess) return hip_fail(e, "fft");
+    // FFT passes (second stream) concurrent with the binning
+    e = ex.fork_join(
+        [&](const Exec &ea) {
+          return ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)modes_h, false); });
+        },
+        [&] { return ex.bin(b, pa, nullptr, gm); });
+    if (e != hipSuccess) return hip_fail(e, "fft / binning");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
       return launch_grad3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(), (const float2 *)u,
                            grad[0], grad[1], grad[2], b.g, p->poly, p->d_poly,
