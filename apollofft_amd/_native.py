@@ -46,7 +46,7 @@ EXPORTS = [
 ]
 
 SPREAD_AUTO, SPREAD_VALU, SPREAD_MFMA = 0, 1, 2
-STAGES = ["fill", "bin", "spread", "fft", "interp", "grad", "prep"]
+STAGES = ["fill", "bin", "spread", "fft", "interp", "grad", "prep", "tile_max"]
 
 
 class NufftError(RuntimeError):

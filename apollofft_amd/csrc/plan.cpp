@@ -1015,7 +1015,7 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
                                     p->opts.compat_reference_type2_3d_scaling != 0);
           });
           if (r != hipSuccess) return r;
-          return ea.timed(ANUFFT_STAGE_INTERP, [&] {
+          return ea.timed(ANUFFT_STAGE_TILE_MAX, [&] {
             return launch_tile_max3d(p->w, p->kernel_method, ea.grid(),
                                      (float *)(ea.ws + lay.tmax), p->bi.g, ea.s);
           });

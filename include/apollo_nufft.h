@@ -242,7 +242,8 @@ typedef enum anufft_stage {
   ANUFFT_STAGE_INTERP = 4, /* Type-2 interpolation */
   ANUFFT_STAGE_GRAD = 5,   /* coordinate-gradient interpolation */
   ANUFFT_STAGE_PREP = 6,   /* per-point weight records (vector-ALU kernels) */
-  ANUFFT_STAGE_COUNT = 7
+  ANUFFT_STAGE_TILE_MAX = 7, /* tile maxima of the grid (MFMA interpolation scales) */
+  ANUFFT_STAGE_COUNT = 8
 } anufft_stage;
 anufft_status anufft_timing_enable(const anufft_plan *plan, int32_t on);
 anufft_status anufft_timing_collect(const anufft_plan *plan, double *ms, int64_t *calls);
