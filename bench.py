@@ -403,7 +403,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this NUFFT build runs on one GPU; multi-GPU sharding is "
-                                   "designed but not built (DESIGN.md), use --gpus 1"}))
+                                   "designed but not built here, use --gpus 1"}))
         sys.exit(2)
 
     import torch
