@@ -235,6 +235,57 @@ def test_validation_report_passes(dev):
     assert r["passed"] is True
 
 
+def test_one_plan_two_streams_concurrently(dev):
+    """Type-1 and Type-2 calls on one plan from two streams with their own
+    workspaces, enqueued from two host threads: the per-call fork/join of the
+    plan's second stream keeps each call's results equal to a serial run."""
+    import threading
+    import torch
+    import apollofft_amd._native as N
+    rng = np.random.default_rng(51)
+    n = (24, 20, 16)
+    plan, d = _plan3(n, w=6)
+    M = 20000
+    xyz = [_t(uniform_points(rng, M), dev) for _ in range(2)]
+    c = [_t(cnormal(rng, M), dev) for _ in range(2)]
+    f = [_t(cnormal(rng, n), dev) for _ in range(2)]
+    ref1 = [plan.type1(xyz[i], c[i]).clone() for i in range(2)]
+    ref2 = [plan.type2(xyz[i], f[i]).clone() for i in range(2)]
+    torch.cuda.synchronize()
+    need = plan.workspace_size(M)
+    ws = [torch.empty(need, dtype=torch.uint8, device=dev) for _ in range(2)]
+    streams = [torch.cuda.Stream(device=dev) for _ in range(2)]
+    out1 = [torch.empty(n, dtype=torch.complex64, device=dev) for _ in range(2)]
+    out2 = [torch.empty(M, dtype=torch.complex64, device=dev) for _ in range(2)]
+    errors = []
+
+    def run(i):
+        try:
+            b = xyz[i].data_ptr()
+            ptrs = N.ptr_array([b, b + 4, b + 8])
+            sp = ctypes.c_void_p(streams[i].cuda_stream)
+            for _ in range(5):
+                N.check(N.lib().anufft_execute_type1(plan.handle, M, ptrs, 3, c[i].data_ptr(),
+                                                     out1[i].data_ptr(), ws[i].data_ptr(),
+                                                     ws[i].numel(), sp))
+                N.check(N.lib().anufft_execute_type2(plan.handle, M, ptrs, 3, f[i].data_ptr(),
+                                                     out2[i].data_ptr(), ws[i].data_ptr(),
+                                                     ws[i].numel(), sp))
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for i in range(2):
+        assert rel_l2(out1[i].cpu().numpy(), ref1[i].cpu().numpy()) < 1e-6
+        assert rel_l2(out2[i].cpu().numpy(), ref2[i].cpu().numpy()) < 1e-6
+
+
 def test_type2_3d_compat_scaling(dev, oracle_mod):
     """compat flag reproduces the reference's direct/(mx my mz) output
     (dimension_3d.rs:549-551)."""
