@@ -207,7 +207,7 @@ struct anufft_plan {
   int blue_n[3] = {0, 0, 0};        // Bluestein length per axis (0: direct mixed radix)
   float2 *d_chirp[3] = {nullptr, nullptr, nullptr};
   float2 *d_bhat[3] = {nullptr, nullptr, nullptr};
-  KbPoly *d_poly = nullptr;         // device copy of poly (scalar-loaded by the MFMA kernels)
+  KbPoly *d_poly = nullptr;         // device copy of poly (read by the MFMA kernels)
   KbPoly poly{};
   double poly_err = 0;
   // Binning geometries: spreading (register columns) and interpolation /

@@ -1,5 +1,6 @@
 // spread.hip -- Kaiser-Bessel spreading (Type-1), interpolation (Type-2) and
-// the point-coordinate gradient, 3D subproblems staged in LDS plus 1D paths.
+// the point-coordinate gradient: 3D subproblems (up to max_sub points of one
+// tile) on the matrix cores or the vector ALUs, plus the 1D paths.
 //
 // Replaces the per-point T^3 loops of apollo-nufft (type1_into spread,
 // transform/dimension_3d.rs:290-327; type2_into gather, :560-601; 1D
@@ -7,26 +8,27 @@
 // fast_type1_spread_3d / fast_type2_interpolate_3d (apollo-nufft-wgpu
 // .../shaders/nufft_fast_3d.wgsl:130-161, :268-299).
 //
-// 3D work unit = subproblem: up to max_sub points of one tile.
+// Matrix-core kernels (default): a tile is PX x cells x 32 y x 32 z.  For
+// each x cell the (y,z) plane is one 32 x 32 output of
+// v_mfma_f32_32x32x16_f16 over 16 points: spreading multiplies
+// A[y][k] = wx_k wy_k(y) by B[k][z] = c_k wz_k(z); interpolation multiplies
+// the grid plane by wz_k(z) and contracts y and x on the vector ALUs.  FP32
+// operands are split into FP16 hi + lo (three products per FP32 product,
+// FP32 accumulation), with power-of-two scales per subproblem / tile.
+// Points come from 32-byte records written in sorted order by the binning.
+// A workgroup walks a contiguous, work-balanced run of the subproblem list
+// (pencil order); spreading keeps its accumulators as a ring of x columns
+// and flushes only the columns a tile leaves behind.
 //
-// Spreading (Type-1) is output-stationary: a 1024-thread workgroup owns the
-// padded tile of a subproblem, lane (py, pz) of the workgroup owns one (y,z)
-// column of it (32 x 32 columns; each wave owns an 8 x 8 block of columns)
-// and keeps the column's bin_x + 2w x-cells in registers.  Points are staged
-// 256 at a time in LDS (coordinates -> grid units in FP64, 2w FP32 tap
-// weights per axis from the per-tap polynomials, value, tap origin); every
-// wave walks the staged points and skips, with one uniform test, those whose
-// (y,z) support misses its column block; otherwise each lane multiplies the
-// point's value by its (y,z) weight and adds the 2w x-taps into its
-// registers (the x origin is uniform, so a switch selects statically indexed
-// registers).  Nothing is accumulated through LDS atomics: on gfx950 the
-// LDS FP32 atomic add measured 0.33 lane-ops/clk/CU, FP64 7.8, against ~64
-// lane-FMAs/clk/CU for register FMAs.  At the end each lane adds its column
-// to the HBM grid with global float atomics (periodic wrap, zeros skipped).
-//
-// Interpolation (Type-2) loads its own, smaller padded tile from HBM into
-// LDS once per subproblem; each wave gathers one point at a time (lanes over
-// (y,z) taps, loop over x) and reduces across the wave.
+// Vector-ALU kernels: spreading is output-stationary: a 1024-thread
+// workgroup owns the padded tile of a subproblem, lane (py, pz) owns one
+// (y,z) column and keeps the column's bin_x + 2w x-cells in registers;
+// points are staged 256 at a time in LDS and each lane adds the 2w x-taps of
+// the points whose support meets its column.  Nothing is accumulated through
+// LDS atomics: on gfx950 the LDS FP32 atomic add measured 0.33
+// lane-ops/clk/CU, FP64 7.8, against ~64 lane-FMAs/clk/CU for register
+// FMAs.  Interpolation loads its padded tile into LDS once per subproblem;
+// each wave gathers one point at a time and reduces across the wave.
 #include "internal.h"
 
 namespace anufft {
