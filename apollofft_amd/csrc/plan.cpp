@@ -1268,14 +1268,21 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   });
   if (e != hipSuccess) return hip_fail(e, "fill");
   const bool mf = p0->kernel_method == kSpreadMfma;
-  e = e0.timed(ANUFFT_STAGE_BIN, [&] {
-    for (int i = 0; i < nprob; ++i) {
+  // the problems' binnings are independent: odd problems on the first
+  // plan's second stream, even ones on the caller's (small problems are
+  // launch-bound, so two streams keep more of them in flight)
+  auto sort_range = [&](int parity, hipStream_t st) {
+    for (int i = parity; i < nprob; i += 2) {
       const RecordOut ro{probs[i].zr, ex[i].zrec(num_points[i], true), (const float2 *)values[i]};
       hipError_t r = sort_points(probs[i].pa, plans[i]->bs.g, 3, ex[i].sort_buffers(plans[i]->bs),
-                                 s, mf ? &ro : nullptr);
+                                 st, mf ? &ro : nullptr);
       if (r != hipSuccess) return r;
     }
     return hipSuccess;
+  };
+  e = e0.timed(ANUFFT_STAGE_BIN, [&] {
+    return e0.fork_join([&](const Exec &ea) { return sort_range(1, ea.s); },
+                        [&] { return sort_range(0, s); });
   });
   if (e != hipSuccess) return hip_fail(e, "binning");
   if (!mf) {
