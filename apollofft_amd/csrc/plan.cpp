@@ -1259,18 +1259,11 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   uint32_t *prefix = (uint32_t *)((char *)ws + off);
   hipError_t e;
   const Exec &e0 = ex[0];
-  e = e0.timed(ANUFFT_STAGE_FILL, [&] {
-    for (int i = 0; i < nprob; ++i) {
-      hipError_t r = launch_fill(ex[i].grid(), plans[i]->cells(), s);
-      if (r != hipSuccess) return r;
-    }
-    return hipSuccess;
-  });
-  if (e != hipSuccess) return hip_fail(e, "fill");
   const bool mf = p0->kernel_method == kSpreadMfma;
-  // the problems' binnings are independent: odd problems on the first
-  // plan's second stream, even ones on the caller's (small problems are
-  // launch-bound, so two streams keep more of them in flight)
+  // the grids' zeroing and the problems' binnings are independent: the
+  // zeroing and the odd problems on the first plan's second stream, even
+  // problems on the caller's (small problems are launch-bound, so two
+  // streams keep more of them in flight)
   auto sort_range = [&](int parity, hipStream_t st) {
     for (int i = parity; i < nprob; i += 2) {
       const RecordOut ro{probs[i].zr, ex[i].zrec(num_points[i], true), (const float2 *)values[i]};
@@ -1281,10 +1274,17 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
     return hipSuccess;
   };
   e = e0.timed(ANUFFT_STAGE_BIN, [&] {
-    return e0.fork_join([&](const Exec &ea) { return sort_range(1, ea.s); },
-                        [&] { return sort_range(0, s); });
+    return e0.fork_join(
+        [&](const Exec &ea) {
+          for (int i = 0; i < nprob; ++i) {
+            hipError_t r = launch_fill(ex[i].grid(), plans[i]->cells(), ea.s);
+            if (r != hipSuccess) return r;
+          }
+          return sort_range(1, ea.s);
+        },
+        [&] { return sort_range(0, s); });
   });
-  if (e != hipSuccess) return hip_fail(e, "binning");
+  if (e != hipSuccess) return hip_fail(e, "fill / binning");
   if (!mf) {
     e = e0.timed(ANUFFT_STAGE_PREP, [&] {
       for (int i = 0; i < nprob; ++i) {
