@@ -1302,12 +1302,19 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
                                    p0->poly, p0->d_poly, nblk, s);
   });
   if (e != hipSuccess) return hip_fail(e, "spread");
-  e = e0.timed(ANUFFT_STAGE_FFT, [&] {
-    for (int i = 0; i < nprob; ++i) {
-      hipError_t r = ex[i].grid_to_modes((float2 *)out[i]);
+  // the problems' FFT passes, odd problems on the second stream
+  auto fft_range = [&](int parity, hipStream_t st) {
+    for (int i = parity; i < nprob; i += 2) {
+      Exec ei = ex[i];
+      ei.s = st;
+      hipError_t r = ei.grid_to_modes((float2 *)out[i]);
       if (r != hipSuccess) return r;
     }
     return hipSuccess;
+  };
+  e = e0.timed(ANUFFT_STAGE_FFT, [&] {
+    return e0.fork_join([&](const Exec &ea) { return fft_range(1, ea.s); },
+                        [&] { return fft_range(0, s); });
   });
   if (e != hipSuccess) return hip_fail(e, "fft");
   return ANUFFT_OK;
