@@ -59,8 +59,8 @@ constexpr uint32_t kSubWork = 48;
 struct SortBuffers {
   uint32_t *keys[2];
   uint32_t *idx[2];
-  uint32_t *cnt;       // kRadix * nchunks digit counts, then offsets
-  uint32_t *scan_tmp;  // block sums of the scan
+  uint32_t *cnt;       // kRadix * nchunks digit counts, then offsets within each digit
+  uint32_t *scan_tmp;  // kRadix digit totals
   uint32_t *bin_start; // nbins + 1
   uint32_t *sub_ofs;   // nbins + 2; [nbins] = number of subproblems, [nbins + 1] = total work
   int4 *subs;          // (bin, start, count, work before it)

@@ -265,8 +265,7 @@ Layout layout(const anufft_plan *p, int64_t M) {
     int64_t cnt_n = (int64_t)kRadix * nch;
     l.cnt = take(4 * (size_t)cnt_n);
     const int nb = std::max(p->bs.nbins, std::max(p->bi.nbins, p->bg.nbins));
-    int64_t scan_n = std::max<int64_t>(cnt_n, nb + 1);
-    l.scan = take(4 * (size_t)((scan_n + 1023) / 1024 + 1));
+    l.scan = take(4 * (size_t)kRadix);  // digit totals of a pass
     l.bin_start = take(4 * (size_t)(nb + 1));
     l.sub_ofs = take(4 * (size_t)(nb + 2));
     int64_t nsub = 1;

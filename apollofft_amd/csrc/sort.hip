@@ -9,7 +9,8 @@
 //
 // Pipeline (one stream, no host synchronisation):
 //   keys   : key[j] = tile of point j, per-chunk histogram of digit 0
-//   per digit pass: scan(counts) -> stable scatter (-> histogram of next digit)
+//   per digit pass: per-digit scan of the chunk counts -> stable scatter
+//                   (histogram of the next digit)
 //   bounds : bin_start[b] = lower_bound(sorted keys, b)
 //   subproblems (one block): scan of per-tile subproblem counts, list
 //   subs   : subproblems of at most max_sub points per tile, tiles in pencil
@@ -110,54 +111,35 @@ __device__ __forceinline__ uint32_t block_scan_waves(uint32_t v, uint32_t *sh, u
   return before + x - v;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t *data,
-                                                                   int64_t n, uint32_t *sums) {
-  __shared__ uint32_t sh[kScanThreads];
-  int64_t base = (int64_t)blockIdx.x * kScanTile;
-  uint32_t s = 0;
-  for (int i = 0; i < kScanItems; ++i) {
-    int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
-    if (j < n) s += data[j];
+// Exclusive scan of each digit's row of chunk counts, in place: one block per
+// digit, the row walked in tiles of kScanTile with a running carry;
+// tot[d] = the row's total.  The scatter adds the digit's base (the totals
+// of the smaller digits) itself, so one launch replaces a whole-array scan.
+__global__ __launch_bounds__(kScanThreads) void scan_rows_kernel(uint32_t *cnt, int nchunks,
+                                                                 uint32_t *tot) {
+  __shared__ uint32_t sh[kScanThreads / 64];
+  uint32_t *row = cnt + (int64_t)blockIdx.x * nchunks;
+  uint32_t carry = 0;
+  for (int base = 0; base < nchunks; base += kScanTile) {
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const int j = base + threadIdx.x * kScanItems + i;
+      v[i] = j < nchunks ? row[j] : 0u;
+      s += v[i];
+    }
+    uint32_t total;
+    uint32_t ex = block_scan_waves<kScanThreads>(s, sh, total) + carry;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      const int j = base + threadIdx.x * kScanItems + i;
+      if (j < nchunks) row[j] = ex;
+      ex += v[i];
+    }
+    carry += total;
   }
-  uint32_t total;
-  block_scan_waves<kScanThreads>(s, sh, total);
-  if (threadIdx.x == 0) sums[blockIdx.x] = total;
-}
-
-// Exclusive scan of each tile, offset by the sum of the tile sums before it
-// (read by every block: the tile count stays small, a few thousand at most).
-__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(uint32_t *data, int64_t n,
-                                                                  const uint32_t *sums) {
-  __shared__ uint32_t sh[kScanThreads];
-  uint32_t before = 0;
-  for (uint32_t i = threadIdx.x; i < blockIdx.x; i += kScanThreads) before += sums[i];
-  uint32_t boff;
-  block_scan_waves<kScanThreads>(before, sh, boff);
-  int64_t base = (int64_t)blockIdx.x * kScanTile;
-  uint32_t v[kScanItems];
-  uint32_t s = 0;
-  for (int i = 0; i < kScanItems; ++i) {
-    int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
-    v[i] = j < n ? data[j] : 0u;
-    s += v[i];
-  }
-  uint32_t total;
-  uint32_t ex = block_scan_waves<kScanThreads>(s, sh, total) + boff;
-  for (int i = 0; i < kScanItems; ++i) {
-    int64_t j = base + (int64_t)threadIdx.x * kScanItems + i;
-    if (j < n) data[j] = ex;
-    ex += v[i];
-  }
-}
-
-hipError_t scan_exclusive(uint32_t *data, int64_t n, uint32_t *tmp, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  int64_t nb = (n + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
-                     tmp);
-  hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s, data, n,
-                     tmp);
-  return hipGetLastError();
+  if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
 // ---- stable scatter of one digit --------------------------------------------
@@ -179,14 +161,19 @@ struct RecMove {
 template <bool REC>
 __global__ __launch_bounds__(kSortThreads) void scatter_kernel(
     const uint32_t *keys_in, const uint32_t *idx_in, uint32_t *keys_out, uint32_t *idx_out,
-    int64_t n, int shift, const uint32_t *offs, int nchunks, RecMove rm) {
+    int64_t n, int shift, const uint32_t *offs, const uint32_t *tot, int nchunks, RecMove rm) {
   constexpr int kWaves = kSortThreads / 64;
+  static_assert(kSortThreads == kRadix, "one thread per digit");
   __shared__ uint32_t running[kRadix];
   __shared__ uint32_t wcount[kWaves][kRadix];
+  __shared__ uint32_t sh[kWaves];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int d = tid; d < kRadix; d += kSortThreads) {
-    running[d] = offs[(int64_t)d * nchunks + blockIdx.x];
-    for (int w = 0; w < kWaves; ++w) wcount[w][d] = 0;
+  {
+    // start of digit tid in this chunk: digits below it, then its earlier chunks
+    uint32_t all;
+    const uint32_t dbase = block_scan_waves<kSortThreads>(tot[tid], sh, all);
+    running[tid] = dbase + offs[(int64_t)tid * nchunks + blockIdx.x];
+    for (int w = 0; w < kWaves; ++w) wcount[w][tid] = 0;
   }
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -328,8 +315,8 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
         hipLaunchKernelGGL(hist_kernel, dim3(sb.nchunks), dim3(kSortThreads), 0, s, ks, n,
                            p * kRadixBits, sb.cnt, sb.nchunks);
       }
-      hipError_t e = scan_exclusive(sb.cnt, (int64_t)kRadix * sb.nchunks, sb.scan_tmp, s);
-      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(scan_rows_kernel, dim3(kRadix), dim3(kScanThreads), 0, s, sb.cnt,
+                         sb.nchunks, sb.scan_tmp);
       RecMove rm{};
       if (rec) {
         rm.out = dst(p);
@@ -340,12 +327,13 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
         rm.first = p == 0;
         hipLaunchKernelGGL(scatter_kernel<true>, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
                            sb.keys[0], (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                           (uint32_t *)nullptr, n, p * kRadixBits, sb.cnt, sb.nchunks, rm);
+                           (uint32_t *)nullptr, n, p * kRadixBits, sb.cnt, sb.scan_tmp,
+                           sb.nchunks, rm);
       } else {
         hipLaunchKernelGGL(scatter_kernel<false>, dim3(sb.nchunks), dim3(kSortThreads), 0, s,
                            sb.keys[p & 1], p == 0 ? (const uint32_t *)nullptr : sb.idx[p & 1],
                            sb.keys[(p + 1) & 1], sb.idx[(p + 1) & 1], n, p * kRadixBits, sb.cnt,
-                           sb.nchunks, rm);
+                           sb.scan_tmp, sb.nchunks, rm);
       }
     }
   }
