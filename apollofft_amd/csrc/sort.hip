@@ -263,13 +263,18 @@ __device__ __forceinline__ int pencil_bin(int q, int3 nb) {
 // tiles.  subs[i].w = work before subproblem i in list order, in units of
 // kWorkUnit points, a subproblem weighing ceil(count / kWorkUnit) +
 // kSubWork (its fixed cost); nsub[0] = number of subproblems, nsub[1] =
-// total work.
+// total work.  The entries of a round are written by the whole block, each
+// finding its tile by a binary search of the tiles' first entries (a tile of
+// a dense cluster can hold hundreds of subproblems).
 constexpr int kSubThreads = 1024;
 __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t *bin_start,
                                                                   int nbins, int3 nb, int max_sub,
                                                                   uint32_t *nsub, int4 *subs) {
-  __shared__ uint32_t sh[kSubThreads];
+  __shared__ uint32_t sh[kSubThreads / 64];
+  __shared__ uint32_t first[kSubThreads];  // first entry of each tile of the round
+  __shared__ int4 tinfo[kSubThreads];      // tile: bin, first point, count, work before it
   const uint32_t ms = (uint32_t)max_sub;
+  const uint32_t wfull = (ms + kWorkUnit - 1) / kWorkUnit + kSubWork;  // a full subproblem
   uint32_t carry = 0, carry_w = 0;
   for (int base = 0; base < nbins; base += kSubThreads) {
     const int q = base + threadIdx.x;
@@ -283,12 +288,24 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t
       wk = (c + kWorkUnit - 1) / kWorkUnit + k * kSubWork;
     }
     uint32_t total, total_w;
-    const uint32_t o = carry + block_scan_waves<kSubThreads>(k, sh, total);
-    uint32_t w = carry_w + block_scan_waves<kSubThreads>(wk, sh, total_w);
-    for (uint32_t t = 0; t < k; ++t) {
-      const uint32_t cnt = min(c - t * ms, ms);
-      subs[o + t] = make_int4(b, (int)(start + t * ms), (int)cnt, (int)w);
-      w += (cnt + kWorkUnit - 1) / kWorkUnit + kSubWork;
+    const uint32_t o = block_scan_waves<kSubThreads>(k, sh, total);
+    const uint32_t w = carry_w + block_scan_waves<kSubThreads>(wk, sh, total_w);
+    first[threadIdx.x] = o;
+    tinfo[threadIdx.x] = make_int4(b, (int)start, (int)c, (int)w);
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < total; t += kSubThreads) {
+      // owner = last tile with first <= t; it has t - first < its count
+      int lo = 0, hi = kSubThreads - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      const int4 ti = tinfo[lo];
+      const uint32_t tl = t - first[lo];
+      const uint32_t cnt = min((uint32_t)ti.z - tl * ms, ms);
+      subs[carry + t] = make_int4(ti.x, (int)((uint32_t)ti.y + tl * ms), (int)cnt,
+                                  (int)((uint32_t)ti.w + tl * wfull));
     }
     carry += total;
     carry_w += total_w;
