@@ -1061,7 +1061,7 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
         [&](const Exec &ea) {
           return ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)modes_h, false); });
         },
-        [&] { return ex.bin(b, pa, nullptr, gm); });
+        [&] { return ex.bin(b, pa, (const float2 *)u, gm); });
     if (e != hipSuccess) return hip_fail(e, "fft / binning");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
       return launch_grad3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(), (const float2 *)u,

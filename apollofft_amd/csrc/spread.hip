@@ -1210,7 +1210,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
 // wz'_k(z) (two MFMA products per x cell), the three derivative sums are
 //   Sx_k = sum_i wx'_k(i) sum_y wy_k(y) D_i,  Sy_k = sum_i wx_k(i) sum_y wy'_k(y) D_i,
 //   Sz_k = sum_i wx_k(i) sum_y wy_k(y) E_i,
-// and grad_a = -scale_a Re(conj(u_k) S_a).  Two x cells per wave (the
+// and grad_a = -scale_a Re(conj(u_k) S_a), u_k read from the point's record
+// (the binning stores u in the value fields).  Two x cells per wave (the
 // second B operand and the derivative rows need the registers).
 template <int W>
 struct MGrad {
@@ -1229,14 +1230,15 @@ struct GLds {
   static constexpr int ZS = JIDX + kGNP * 4;                       // float [3][kGNP]
   static constexpr int OFS = ZS + 3 * kGNP * 4;                    // int [kGNP]
   static constexpr int TMX = OFS + kGNP * 4;                       // float [16]
-  static constexpr int SIZE = TMX + 16 * 4;
+  static constexpr int U = TMX + 16 * 4;                           // float2 [kGNP]
+  static constexpr int SIZE = U + kGNP * 8;
 };
 
 
 template <int W>
 __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
-    PointArgs pa, ZRec zr, const float2 *grid, const float2 *u, float *gx, float *gy, float *gz,
-    Geom g, const KbPoly *__restrict__ dpoly) {
+    PointArgs pa, ZRec zr, const float2 *grid, float *gx, float *gy, float *gz, Geom g,
+    const KbPoly *__restrict__ dpoly) {
   using MG = MGrad<W>;
   constexpr int NT = MG::NT, XW = MG::XW, NW = MG::NW;
   using L = GLds<NW>;
@@ -1247,6 +1249,7 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
   float *ZS = (float *)(shm + L::ZS);
   int *OFS = (int *)(shm + L::OFS);
   float *tmx = (float *)(shm + L::TMX);
+  float2 *U = (float2 *)(shm + L::U);
   auto row = [&](int which, int p) { return ROWS + (which * kGNP + p) * kIRow; };
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1309,26 +1312,41 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
       }
     }
 
+    // records of the next round of this subproblem, loaded during the
+    // current round (thread p < kGNP holds point p)
+    float4 nza = make_float4(0.f, 0.f, 0.f, 0.f), nzb = nza;
+    bool have = false;
     for (int c0 = 0; c0 < sub.z; c0 += kGNP) {
       const int npts = min(kGNP, sub.z - c0);
       const int ng = (npts + 31) >> 5;
       const int np32 = ng << 5;
       const int start = sub.y + c0;
       __syncthreads();
-      // pass 0: zero the six staged rows; arguments, offsets, output indices
-      for (int p = threadIdx.x; p < np32; p += NT) {
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-#pragma unroll
-          for (int i = 0; i < 32; i += 4) *(float4 *)(row(r, p) + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+      // pass 0: zero the six staged rows (all threads); arguments, offsets,
+      // output indices and values of the points
+      for (int t = threadIdx.x; t < np32 * 48; t += NT) {
+        const int p = t / 48, r = (t >> 3) % 6, i = (t & 7) << 2;
+        *(float4 *)(row(r, p) + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      if (threadIdx.x < np32) {
+        const int p = threadIdx.x;
         const bool ok = p < npts;
-        const float4 z = ok ? zr.r[2 * (int64_t)(start + p)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 z = make_float4(0.f, 0.f, 0.f, 0.f), c = z;
+        if (ok) {
+          z = have ? nza : zr.r[2 * (int64_t)(start + p)];
+          c = have ? nzb : zr.r[2 * (int64_t)(start + p) + 1];
+        }
         ZS[p] = z.x;
         ZS[kGNP + p] = z.y;
         ZS[2 * kGNP + p] = z.z;
         OFS[p] = __float_as_int(z.w);
-        if (ok) JIDX[p] = (int)__float_as_uint(zr.r[2 * (int64_t)(start + p) + 1].w);
+        JIDX[p] = (int)__float_as_uint(c.w);
+        U[p] = make_float2(c.x, c.y);
+      }
+      have = c0 + kGNP < sub.z;
+      if (have && threadIdx.x < min(kGNP, sub.z - c0 - kGNP)) {
+        nza = zr.r[2 * (int64_t)(start + kGNP + threadIdx.x)];
+        nzb = zr.r[2 * (int64_t)(start + kGNP + threadIdx.x) + 1];
       }
       __syncthreads();
       // pass 1: value and derivative taps, whole taps per wave
@@ -1434,7 +1452,7 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
           sv[c] = a * s_out * (c < 4 ? kProdUnscale : kProdUnscaleD);
         }
         const int j = JIDX[p];
-        const float2 uj = u[j];
+        const float2 uj = U[p];
         // Re(conj(u) * (-s) * S) = -s (u.x S.x + u.y S.y)
         gx[j] = -(float)g.scale[0] * (uj.x * sv[0] + uj.y * sv[1]);
         if (gy) gy[j] = -(float)g.scale[1] * (uj.x * sv[2] + uj.y * sv[3]);
@@ -1842,8 +1860,10 @@ struct Grad3Mfma {
       hipError_t e = hipFuncSetAttribute((const void *)grad3d_mfma_kernel<W>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
       if (e != hipSuccess) return e;
+      // u reaches the kernel through the records' value fields
+      (void)u;
       hipLaunchKernelGGL(grad3d_mfma_kernel<W>, dim3(nblk), dim3(MGrad<W>::NT), lds, s, pa, zr,
-                         grid, u, gx, gy, gz, g, dpoly);
+                         grid, gx, gy, gz, g, dpoly);
       return hipGetLastError();
     } else {
       return hipErrorInvalidValue;  // w = 8 uses the LDS-tile kernel (grad_uses_mfma)
