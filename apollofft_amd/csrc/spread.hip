@@ -1220,7 +1220,7 @@ struct MGrad {
   static constexpr int NW = PX / XW;
   static constexpr int NT = 64 * NW;
 };
-constexpr int kGNP = 64;  // points staged per round (2 MFMA groups of 32)
+constexpr int kGNP = 128;  // points staged per round (4 MFMA groups of 32)
 
 template <int NW>
 struct GLds {
@@ -1232,6 +1232,7 @@ struct GLds {
   static constexpr int TMX = OFS + kGNP * 4;                       // float [16]
   static constexpr int U = TMX + 16 * 4;                           // float2 [kGNP]
   static constexpr int SIZE = U + kGNP * 8;
+  static_assert(SIZE <= 160 * 1024, "one gradient workgroup per CU");
 };
 
 
