@@ -1226,12 +1226,12 @@ template <int NW>
 struct GLds {
   static constexpr int ROWS = 0;                                   // float [6][kGNP][kIRow]
   static constexpr int RED = ROWS + 6 * kGNP * kIRow * 4;          // float [NW][6][kGNP]
-  static constexpr int JIDX = RED + NW * 6 * kGNP * 4;             // int [kGNP]
-  static constexpr int ZS = JIDX + kGNP * 4;                       // float [3][kGNP]
+  static constexpr int JIDX = RED + NW * 6 * kGNP * 4;             // int [2][kGNP]
+  static constexpr int ZS = JIDX + 2 * kGNP * 4;                   // float [3][kGNP]
   static constexpr int OFS = ZS + 3 * kGNP * 4;                    // int [kGNP]
   static constexpr int TMX = OFS + kGNP * 4;                       // float [16]
-  static constexpr int U = TMX + 16 * 4;                           // float2 [kGNP]
-  static constexpr int SIZE = U + kGNP * 8;
+  static constexpr int U = TMX + 16 * 4;                           // float2 [2][kGNP]
+  static constexpr int SIZE = U + 2 * kGNP * 8;
   static_assert(SIZE <= 160 * 1024, "one gradient workgroup per CU");
 };
 
@@ -1256,6 +1256,7 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
   const XcdChunk xr(pa);
+  int rbuf = 0;  // buffer of the round's output indices and values
   for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
@@ -1322,7 +1323,14 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
       const int ng = (npts + 31) >> 5;
       const int np32 = ng << 5;
       const int start = sub.y + c0;
-      __syncthreads();
+      // No barrier here: the staged rows, ZS and OFS were last read before
+      // the previous round's reduction barrier, and the output indices and
+      // values alternate between two buffers, so this round's staging may
+      // run while other waves still write the previous round's outputs
+      // (RED is rewritten only after the next two barriers).
+      int *JX = JIDX + rbuf * kGNP;
+      float2 *UX = U + rbuf * kGNP;
+      rbuf ^= 1;
       // pass 0: zero the six staged rows (all threads); arguments, offsets,
       // output indices and values of the points
       for (int t = threadIdx.x; t < np32 * 48; t += NT) {
@@ -1341,8 +1349,8 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
         ZS[kGNP + p] = z.y;
         ZS[2 * kGNP + p] = z.z;
         OFS[p] = __float_as_int(z.w);
-        JIDX[p] = (int)__float_as_uint(c.w);
-        U[p] = make_float2(c.x, c.y);
+        JX[p] = (int)__float_as_uint(c.w);
+        UX[p] = make_float2(c.x, c.y);
       }
       have = c0 + kGNP < sub.z;
       if (have && threadIdx.x < min(kGNP, sub.z - c0 - kGNP)) {
@@ -1452,8 +1460,8 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
           for (int w2 = 0; w2 < NW; ++w2) a += RED[(w2 * 6 + c) * kGNP + p];
           sv[c] = a * s_out * (c < 4 ? kProdUnscale : kProdUnscaleD);
         }
-        const int j = JIDX[p];
-        const float2 uj = U[p];
+        const int j = JX[p];
+        const float2 uj = UX[p];
         // Re(conj(u) * (-s) * S) = -s (u.x S.x + u.y S.y)
         gx[j] = -(float)g.scale[0] * (uj.x * sv[0] + uj.y * sv[1]);
         if (gy) gy[j] = -(float)g.scale[1] * (uj.x * sv[2] + uj.y * sv[3]);
