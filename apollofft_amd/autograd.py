@@ -4,7 +4,9 @@ apollo-nufft has no autograd).
 Type-2 ``c = T2(f, x)``, c_j = sum_k f_k exp(+2 pi i k.x_j / L):
   grad_f = T1(g) at the same points (the opposite-type NUFFT),
   grad_x[j, a] = Re(conj(g_j) d c_j / d x_j^(a)), evaluated on the oversampled
-  grid with the kernel derivative (anufft_execute_grad_points).
+  grid with the kernel derivative (anufft_execute_grad_points); when both
+  are needed on a plan with the MFMA kernels, one call computes both from a
+  single binning (anufft_execute_type2_backward).
 Type-1 ``F = T1(c, x)``, F_k = sum_j c_j exp(-2 pi i k.x_j / L):
   grad_c = T2(G) at the same points,
   grad_x[j, a] = Re(conj(c_j) d/dx^(a) sum_k G_k exp(+2 pi i k.x_j / L)).
@@ -37,6 +39,10 @@ class _Type2(torch.autograd.Function):
         plan = ctx.plan
         g = g.contiguous()
         gm = gx = None
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1] and plan.fused_type2_backward:
+            # both gradients from one binning of the points
+            gm, gx = plan.type2_backward(positions.contiguous(), g, modes.contiguous())
+            return gm, gx.reshape(positions.shape).to(positions.dtype), None
         if ctx.needs_input_grad[0]:
             gm = plan.type1(positions.contiguous(), g)
         if ctx.needs_input_grad[1]:

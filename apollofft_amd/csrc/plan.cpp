@@ -1080,6 +1080,54 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
   return ANUFFT_OK;
 }
 
+anufft_status anufft_execute_type2_backward(const anufft_plan *p, int64_t M,
+                                            const float *const coords[3], int64_t stride,
+                                            const float *g, const float *modes,
+                                            float *grad_modes, float *const grad_x[3], void *ws,
+                                            size_t ws_bytes, void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, g, modes, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  if (!grad_modes) return fail(ANUFFT_INVALID_ARG, "grad_modes is NULL");
+  if (M > 0 && (!grad_x || !grad_x[0])) return fail(ANUFFT_INVALID_ARG, "grad_x[0] is NULL");
+  if (p->dim != 3 || !grad_uses_mfma(p->w, p->kernel_method))
+    return fail(ANUFFT_UNSUPPORTED, "the fused Type-2 backward needs a 3D plan on the MFMA kernels (w <= 7)");
+  if (ws_bytes < 2 * lay.total)
+    return fail(ANUFFT_CAPACITY_EXCEEDED, "workspace of %zu bytes is smaller than the %zu required",
+                ws_bytes, 2 * lay.total);
+  if (M == 0)  // T1 of no points is zero; no coordinates to differentiate
+    return anufft_execute_type1(p, 0, coords, stride, g, grad_modes, ws, ws_bytes, stream);
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  // the second half of the workspace holds the fine grid of the modes
+  const Exec eh{p, (hipStream_t)stream, (char *)ws + lay.total, lay};
+  // the MFMA gradient bins with the spreading tiles, so one binning (with g
+  // in the records) serves the spreading and the gradient kernel
+  PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+  hipError_t e = ex.fork_join(
+      [&](const Exec &ea) {
+        Exec eb = eh;
+        eb.s = ea.s;
+        hipError_t r = eb.timed(ANUFFT_STAGE_FFT, [&] { return eb.modes_to_grid((const float2 *)modes, false); });
+        if (r == hipSuccess)
+          r = ea.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ea.grid(), p->cells(), ea.s); });
+        return r;
+      },
+      [&] { return ex.bin(p->bs, pa, (const float2 *)g, true); });
+  if (e != hipSuccess) return hip_fail(e, "fft / fill / binning");
+  e = ex.spread3(pa, (const float2 *)g, ex.grid());
+  if (e != hipSuccess) return hip_fail(e, "spread");
+  e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.grid_to_modes((float2 *)grad_modes); });
+  if (e != hipSuccess) return hip_fail(e, "fft");
+  e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
+    return launch_grad3d(p->w, p->kernel_method, pa, ex.zrec(M), eh.grid(), (const float2 *)g,
+                         grad_x[0], grad_x[1], grad_x[2], p->bi.g, p->poly, p->d_poly,
+                         ex.blocks(p->bi, M, 1), ex.s);
+  });
+  if (e != hipSuccess) return hip_fail(e, "grad");
+  return ANUFFT_OK;
+}
+
 anufft_status anufft_timing_enable(const anufft_plan *p, int32_t on) {
   if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
   std::lock_guard<std::mutex> lk(p->timer.mu);

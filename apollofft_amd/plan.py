@@ -91,9 +91,9 @@ class _PlanBase:
                 "workspace_size")
         return b.value
 
-    def workspace(self, num_points, device):
+    def workspace(self, num_points, device, copies=1):
         import torch
-        need = self.workspace_size(num_points)
+        need = copies * self.workspace_size(num_points)
         if self._ws is None or self._ws.numel() < need or self._ws.device != device:
             self._ws = torch.empty(max(need, 256), dtype=torch.uint8, device=device)
         return self._ws
@@ -227,6 +227,32 @@ class _PlanBase:
             self._h, M, N.ptr_array(ptrs), stride, u.data_ptr(), modes_h.data_ptr(),
             N.ptr_array(gp, 3), ws.data_ptr(), ws.numel(), self._stream(dev)), "grad_points")
         return g.t()
+
+    @property
+    def fused_type2_backward(self):
+        """True when anufft_execute_type2_backward serves this plan (3D, MFMA
+        gradient kernel: w <= 7)."""
+        return (self.dim == 3 and self.meta["kernel_method"] == "mfma"
+                and self.meta["kernel_width"] <= 7)
+
+    def type2_backward(self, positions, g, modes):
+        """Backward of c = T2(modes, x) for the upstream gradient g on c:
+        (T1(g) at the points, the coordinate gradient (M, 3)), both from one
+        binning of the points (anufft_execute_type2_backward)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        if g.dtype != torch.complex64 or not g.is_contiguous() or g.numel() != M:
+            raise ValueError("g must be a contiguous complex64 tensor with one entry per point")
+        if modes.dtype != torch.complex64 or not modes.is_contiguous() or tuple(modes.shape) != self.modes_shape:
+            raise ValueError("modes must be a contiguous complex64 tensor of the mode shape")
+        gm = torch.empty(self.modes_shape, dtype=torch.complex64, device=dev)
+        gx = torch.empty((self.dim, M), dtype=torch.float32, device=dev)
+        gp = [gx[a].data_ptr() for a in range(self.dim)] + [None] * (3 - self.dim)
+        ws = self.workspace(M, dev, copies=2)
+        N.check(N.lib().anufft_execute_type2_backward(
+            self._h, M, N.ptr_array(ptrs), stride, g.data_ptr(), modes.data_ptr(), gm.data_ptr(),
+            N.ptr_array(gp, 3), ws.data_ptr(), ws.numel(), self._stream(dev)), "type2_backward")
+        return gm, gx.t()
 
     # ---- host drop-in execution (numpy; copies in and out) -------------------------
     def _host_coords(self, positions):

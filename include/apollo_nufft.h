@@ -159,6 +159,25 @@ anufft_status anufft_execute_grad_points(const anufft_plan *plan, int64_t num_po
                                          float *const grad[3], void *workspace,
                                          size_t workspace_bytes, void *hip_stream);
 
+/* Backward of a 3D Type-2 c = T2(f, x) for an upstream gradient g on c (no
+ * reference counterpart; the reference has no autograd): writes
+ *   grad_modes = T1(g, x)                                  (mode shape)
+ *   grad_x[a]  = anufft_execute_grad_points(x, u = g, h = f)[a]
+ * from ONE binning of the points carrying g (the Type-1 and the gradient
+ * then share the records), with the fine grid of f built on the plan's
+ * second stream during the binning.  Same results as the two separate
+ * calls.  3D plans whose gradient runs on the MFMA kernels (kernel_method
+ * 0 or 2, w <= 7); other plans return ANUFFT_UNSUPPORTED.  grad_modes must
+ * be non-NULL, grad_x[0] too when num_points > 0 (grad_x[1], grad_x[2] may be
+ * NULL).  The workspace is twice anufft_workspace_size(plan, num_points)
+ * bytes. */
+anufft_status anufft_execute_type2_backward(const anufft_plan *plan, int64_t num_points,
+                                            const float *const coords[3], int64_t coord_stride,
+                                            const float *g, const float *modes,
+                                            float *grad_modes, float *const grad_x[3],
+                                            void *workspace, size_t workspace_bytes,
+                                            void *hip_stream);
+
 /* Batched Type-1 over independent 3D plans (no reference counterpart: the
  * reference has no batched NUFFT).  Problem p uses plans[p], num_points[p],
  * coords_p[a] = coords[3p + a] (stride coord_stride), values[p], out[p].  The

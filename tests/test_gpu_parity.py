@@ -432,6 +432,46 @@ def test_grad_points_type2_vs_direct(dev, oracle_mod, method, w):
     assert rel_l2(out.cpu().numpy(), ref) < TOL
 
 
+@pytest.mark.parametrize("w", [4, 7])
+def test_type2_backward_matches_separate_calls(dev, oracle_mod, w):
+    """The fused Type-2 backward (one binning) against the Type-1 of g and the
+    coordinate gradient computed by separate calls, and against FP64 direct
+    sums; an odd grid, clustered points and small subproblems (several rounds
+    and subproblems per tile); M = 0; and the unsupported vector-ALU plan."""
+    import torch
+    import apollofft_amd._native as N
+    rng = np.random.default_rng(61)
+    n = (20, 13, 16)
+    plan, d = _plan3(n, w=w, method="mfma", max_sub=300)
+    M = 6000
+    xyz = uniform_points(rng, M)
+    xyz[: M // 2] = (0.3 + 0.05 * rng.standard_normal((M // 2, 3))).astype(np.float32)
+    f = cnormal(rng, n)
+    g = cnormal(rng, M)
+    tx, tg, tf = _t(xyz, dev), _t(g, dev), _t(f, dev)
+    assert plan.fused_type2_backward
+    gm, gx = plan.type2_backward(tx, tg, tf)
+    gm_sep = plan.type1(tx, tg)
+    gx_sep = plan.grad_points(tx, tg, tf)
+    torch.cuda.synchronize()
+    assert rel_l2(gm.cpu().numpy(), gm_sep.cpu().numpy()) < 1e-6
+    assert rel_l2(gx.cpu().numpy(), gx_sep.cpu().numpy()) < 1e-6
+    x64 = xyz.astype(np.float64)
+    ref_m = oracle_mod.direct_type1_3d(x64, g.astype(np.complex128), n, d)
+    assert rel_l2(gm.cpu().numpy(), ref_m) < TOL
+    pj = rng.choice(M, size=300, replace=False)
+    ref_x = oracle_mod.direct_type2_3d_grad_x(f.astype(np.complex128), x64[pj], d,
+                                              g[pj].astype(np.complex128) / 2.0)
+    assert rel_l2(gx.cpu().numpy()[pj], ref_x) < TOL
+    gm0, gx0 = plan.type2_backward(tx[:0], tg[:0], tf)
+    torch.cuda.synchronize()
+    assert gx0.shape == (0, 3) and float(gm0.abs().max()) == 0.0
+    pv, _ = _plan3(n, w=w, method="valu")
+    assert not pv.fused_type2_backward
+    with pytest.raises(N.Unsupported):
+        pv.type2_backward(tx, tg, tf)
+
+
 # ---------------------------------------------------------------- host drop-in API
 def test_host_api_matches_direct(dev, oracle_mod):
     import apollofft_amd as A
