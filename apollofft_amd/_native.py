@@ -38,6 +38,7 @@ EXPORTS = [
     "anufft_plan_destroy", "anufft_plan_get_meta", "anufft_plan_get_deconv",
     "anufft_workspace_size", "anufft_execute_type1", "anufft_execute_type2",
     "anufft_execute_grad_points", "anufft_execute_type2_backward",
+    "anufft_execute_type1_backward",
     "anufft_batch_workspace_size",
     "anufft_execute_type1_batched", "anufft_execute_type1_host",
     "anufft_execute_type2_host", "anufft_direct_type1", "anufft_direct_type2",
@@ -161,6 +162,8 @@ def lib():
         "anufft_execute_type2": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
         "anufft_execute_grad_points": (i32, [vp, i64, P(vp), i64, vp, vp, P(vp), vp, sz, vp]),
         "anufft_execute_type2_backward": (i32, [vp, i64, P(vp), i64, vp, vp, vp, P(vp), vp, sz,
+                                                vp]),
+        "anufft_execute_type1_backward": (i32, [vp, i64, P(vp), i64, vp, vp, vp, P(vp), vp, sz,
                                                 vp]),
         "anufft_batch_workspace_size": (i32, [P(vp), i32, P(i64), P(sz)]),
         "anufft_execute_type1_batched": (i32, [P(vp), i32, P(i64), P(vp), i64, P(vp), P(vp),

@@ -6,7 +6,8 @@ Type-2 ``c = T2(f, x)``, c_j = sum_k f_k exp(+2 pi i k.x_j / L):
   grad_x[j, a] = Re(conj(g_j) d c_j / d x_j^(a)), evaluated on the oversampled
   grid with the kernel derivative (anufft_execute_grad_points); when both
   are needed on a plan with the MFMA kernels, one call computes both from a
-  single binning (anufft_execute_type2_backward).
+  single binning (anufft_execute_type2_backward); likewise for Type-1
+  (anufft_execute_type1_backward: one FFT of G, one binning).
 Type-1 ``F = T1(c, x)``, F_k = sum_j c_j exp(-2 pi i k.x_j / L):
   grad_c = T2(G) at the same points,
   grad_x[j, a] = Re(conj(c_j) d/dx^(a) sum_k G_k exp(+2 pi i k.x_j / L)).
@@ -39,7 +40,7 @@ class _Type2(torch.autograd.Function):
         plan = ctx.plan
         g = g.contiguous()
         gm = gx = None
-        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1] and plan.fused_type2_backward:
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1] and plan.fused_backward:
             # both gradients from one binning of the points
             gm, gx = plan.type2_backward(positions.contiguous(), g, modes.contiguous())
             return gm, gx.reshape(positions.shape).to(positions.dtype), None
@@ -66,6 +67,10 @@ class _Type1(torch.autograd.Function):
         plan = ctx.plan
         G = G.contiguous()
         gv = gx = None
+        if ctx.needs_input_grad[0] and ctx.needs_input_grad[1] and plan.fused_backward:
+            # both gradients from one FFT of G and one binning of the points
+            gv, gx = plan.type1_backward(positions.contiguous(), values.contiguous(), G)
+            return gv, gx.reshape(positions.shape).to(positions.dtype), None
         if ctx.needs_input_grad[0]:
             gv = plan.type2(positions.contiguous(), G)
         if ctx.needs_input_grad[1]:

@@ -1128,6 +1128,51 @@ anufft_status anufft_execute_type2_backward(const anufft_plan *p, int64_t M,
   return ANUFFT_OK;
 }
 
+anufft_status anufft_execute_type1_backward(const anufft_plan *p, int64_t M,
+                                            const float *const coords[3], int64_t stride,
+                                            const float *values, const float *G,
+                                            float *grad_values, float *const grad_x[3], void *ws,
+                                            size_t ws_bytes, void *stream) {
+  Layout lay;
+  anufft_status st = check_exec(p, M, coords, values, G, ws, ws_bytes, lay);
+  if (st != ANUFFT_OK) return st;
+  if (M > 0 && !grad_values) return fail(ANUFFT_INVALID_ARG, "grad_values is NULL");
+  if (M > 0 && (!grad_x || !grad_x[0])) return fail(ANUFFT_INVALID_ARG, "grad_x[0] is NULL");
+  if (p->dim != 3 || !grad_uses_mfma(p->w, p->kernel_method))
+    return fail(ANUFFT_UNSUPPORTED, "the fused Type-1 backward needs a 3D plan on the MFMA kernels (w <= 7)");
+  if (M == 0) return ANUFFT_OK;
+  DeviceGuard guard(p->device);
+  Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
+  // T2(G) and the gradient both read the fine grid of G and bin the points
+  // with the same tiles: one FFT (second stream) and one binning carrying
+  // the values c (ignored by the interpolation, u of the gradient)
+  PointArgs pa = ex.point_args(p->bi, M, coords, stride);
+  hipError_t e = ex.fork_join(
+      [&](const Exec &ea) {
+        hipError_t r = ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)G, false); });
+        if (r != hipSuccess) return r;
+        return ea.timed(ANUFFT_STAGE_TILE_MAX, [&] {
+          return launch_tile_max3d(p->w, p->kernel_method, ea.grid(), (float *)(ea.ws + lay.tmax),
+                                   p->bi.g, ea.s);
+        });
+      },
+      [&] { return ex.bin(p->bi, pa, (const float2 *)values, true); });
+  if (e != hipSuccess) return hip_fail(e, "fft / binning");
+  e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
+    return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(),
+                           (float *)(ex.ws + lay.tmax), (float2 *)grad_values, p->bi.g, p->poly,
+                           p->d_poly, ex.blocks(p->bi, M, 2), ex.s);
+  });
+  if (e != hipSuccess) return hip_fail(e, "interp");
+  e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
+    return launch_grad3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(), (const float2 *)values,
+                         grad_x[0], grad_x[1], grad_x[2], p->bi.g, p->poly, p->d_poly,
+                         ex.blocks(p->bi, M, 1), ex.s);
+  });
+  if (e != hipSuccess) return hip_fail(e, "grad");
+  return ANUFFT_OK;
+}
+
 anufft_status anufft_timing_enable(const anufft_plan *p, int32_t on) {
   if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
   std::lock_guard<std::mutex> lk(p->timer.mu);

@@ -229,9 +229,9 @@ class _PlanBase:
         return g.t()
 
     @property
-    def fused_type2_backward(self):
-        """True when anufft_execute_type2_backward serves this plan (3D, MFMA
-        gradient kernel: w <= 7)."""
+    def fused_backward(self):
+        """True when anufft_execute_type{1,2}_backward serve this plan (3D,
+        MFMA gradient kernel: w <= 7)."""
         return (self.dim == 3 and self.meta["kernel_method"] == "mfma"
                 and self.meta["kernel_width"] <= 7)
 
@@ -253,6 +253,25 @@ class _PlanBase:
             self._h, M, N.ptr_array(ptrs), stride, g.data_ptr(), modes.data_ptr(), gm.data_ptr(),
             N.ptr_array(gp, 3), ws.data_ptr(), ws.numel(), self._stream(dev)), "type2_backward")
         return gm, gx.t()
+
+    def type1_backward(self, positions, values, G):
+        """Backward of F = T1(values, x) for the upstream gradient G on F:
+        (T2(G) at the points, the coordinate gradient (M, 3)), from one FFT of
+        G and one binning (anufft_execute_type1_backward)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        if values.dtype != torch.complex64 or not values.is_contiguous() or values.numel() != M:
+            raise ValueError("values must be a contiguous complex64 tensor with one entry per point")
+        if G.dtype != torch.complex64 or not G.is_contiguous() or tuple(G.shape) != self.modes_shape:
+            raise ValueError("G must be a contiguous complex64 tensor of the mode shape")
+        gv = torch.empty(M, dtype=torch.complex64, device=dev)
+        gx = torch.empty((self.dim, M), dtype=torch.float32, device=dev)
+        gp = [gx[a].data_ptr() for a in range(self.dim)] + [None] * (3 - self.dim)
+        ws = self.workspace(M, dev)
+        N.check(N.lib().anufft_execute_type1_backward(
+            self._h, M, N.ptr_array(ptrs), stride, values.data_ptr(), G.data_ptr(), gv.data_ptr(),
+            N.ptr_array(gp, 3), ws.data_ptr(), ws.numel(), self._stream(dev)), "type1_backward")
+        return gv, gx.t()
 
     # ---- host drop-in execution (numpy; copies in and out) -------------------------
     def _host_coords(self, positions):

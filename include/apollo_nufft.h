@@ -178,6 +178,22 @@ anufft_status anufft_execute_type2_backward(const anufft_plan *plan, int64_t num
                                             void *workspace, size_t workspace_bytes,
                                             void *hip_stream);
 
+/* Backward of a 3D Type-1 F = T1(c, x) for an upstream gradient G on F (no
+ * reference counterpart; the reference has no autograd): writes
+ *   grad_values = T2(G, x)                 (M complex, direct-sum scaling)
+ *   grad_x[a]   = anufft_execute_grad_points(x, u = c, h = G)[a]
+ * from one FFT of G and one binning of the points carrying c.  Same results
+ * as the two separate calls (with the direct-sum scaling).  3D plans whose
+ * gradient runs on the MFMA kernels (kernel_method 0 or 2, w <= 7); other
+ * plans return ANUFFT_UNSUPPORTED.  grad_values and grad_x[0] must be
+ * non-NULL when num_points > 0.  Workspace: anufft_workspace_size bytes. */
+anufft_status anufft_execute_type1_backward(const anufft_plan *plan, int64_t num_points,
+                                            const float *const coords[3], int64_t coord_stride,
+                                            const float *values, const float *G,
+                                            float *grad_values, float *const grad_x[3],
+                                            void *workspace, size_t workspace_bytes,
+                                            void *hip_stream);
+
 /* Batched Type-1 over independent 3D plans (no reference counterpart: the
  * reference has no batched NUFFT).  Problem p uses plans[p], num_points[p],
  * coords_p[a] = coords[3p + a] (stride coord_stride), values[p], out[p].  The

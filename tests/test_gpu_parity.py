@@ -449,7 +449,7 @@ def test_type2_backward_matches_separate_calls(dev, oracle_mod, w):
     f = cnormal(rng, n)
     g = cnormal(rng, M)
     tx, tg, tf = _t(xyz, dev), _t(g, dev), _t(f, dev)
-    assert plan.fused_type2_backward
+    assert plan.fused_backward
     gm, gx = plan.type2_backward(tx, tg, tf)
     gm_sep = plan.type1(tx, tg)
     gx_sep = plan.grad_points(tx, tg, tf)
@@ -467,9 +467,47 @@ def test_type2_backward_matches_separate_calls(dev, oracle_mod, w):
     torch.cuda.synchronize()
     assert gx0.shape == (0, 3) and float(gm0.abs().max()) == 0.0
     pv, _ = _plan3(n, w=w, method="valu")
-    assert not pv.fused_type2_backward
+    assert not pv.fused_backward
     with pytest.raises(N.Unsupported):
         pv.type2_backward(tx, tg, tf)
+
+
+@pytest.mark.parametrize("w", [4, 7])
+def test_type1_backward_matches_separate_calls(dev, oracle_mod, w):
+    """The fused Type-1 backward (one FFT of G, one binning) against T2(G)
+    and the coordinate gradient from separate calls and FP64 direct sums;
+    clustered points with small subproblems; M = 0; the vector-ALU plan."""
+    import torch
+    import apollofft_amd._native as N
+    rng = np.random.default_rng(62)
+    n = (18, 14, 11)
+    plan, d = _plan3(n, w=w, method="mfma", max_sub=300)
+    M = 5000
+    xyz = uniform_points(rng, M)
+    xyz[: M // 2] = (-1.0 + 0.05 * rng.standard_normal((M // 2, 3))).astype(np.float32)
+    c = cnormal(rng, M)
+    G = cnormal(rng, n)
+    tx, tc, tG = _t(xyz, dev), _t(c, dev), _t(G, dev)
+    assert plan.fused_backward
+    gv, gx = plan.type1_backward(tx, tc, tG)
+    gv_sep = plan.type2(tx, tG)
+    gx_sep = plan.grad_points(tx, tc, tG)
+    torch.cuda.synchronize()
+    assert rel_l2(gv.cpu().numpy(), gv_sep.cpu().numpy()) < 1e-6
+    assert rel_l2(gx.cpu().numpy(), gx_sep.cpu().numpy()) < 1e-6
+    x64 = xyz.astype(np.float64)
+    ref_v = oracle_mod.direct_type2_3d(G.astype(np.complex128), x64, d)
+    assert rel_l2(gv.cpu().numpy(), ref_v) < TOL
+    pj = rng.choice(M, size=300, replace=False)
+    ref_x = oracle_mod.direct_type2_3d_grad_x(G.astype(np.complex128), x64[pj], d,
+                                              c[pj].astype(np.complex128) / 2.0)
+    assert rel_l2(gx.cpu().numpy()[pj], ref_x) < TOL
+    gv0, gx0 = plan.type1_backward(tx[:0], tc[:0], tG)
+    torch.cuda.synchronize()
+    assert gv0.shape == (0,) and gx0.shape == (0, 3)
+    pv, _ = _plan3(n, w=w, method="valu")
+    with pytest.raises(N.Unsupported):
+        pv.type1_backward(tx, tc, tG)
 
 
 # ---------------------------------------------------------------- host drop-in API
