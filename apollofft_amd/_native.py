@@ -42,6 +42,7 @@ EXPORTS = [
     "anufft_batch_workspace_size",
     "anufft_execute_type1_batched", "anufft_execute_type1_host",
     "anufft_execute_type2_host", "anufft_direct_type1", "anufft_direct_type2",
+    "anufft_direct_type1_host", "anufft_direct_type2_host", "anufft_direct_type1_select",
     "anufft_last_error", "anufft_version",
     "anufft_debug_num_bins", "anufft_debug_sort", "anufft_debug_spread",
     "anufft_timing_enable", "anufft_timing_collect",
@@ -172,6 +173,9 @@ def lib():
         "anufft_execute_type2_host": (i32, [vp, i32, i32, i64, P(vp), i64, vp, vp]),
         "anufft_direct_type1": (i32, [vp, i64, P(vp), i64, vp, vp, vp]),
         "anufft_direct_type2": (i32, [vp, i64, P(vp), i64, vp, vp, vp]),
+        "anufft_direct_type1_host": (i32, [vp, i64, P(vp), i64, vp, vp]),
+        "anufft_direct_type2_host": (i32, [vp, i64, P(vp), i64, vp, vp]),
+        "anufft_direct_type1_select": (i32, [vp, i64, P(vp), i64, vp, i64, vp, vp, vp]),
         "anufft_debug_num_bins": (i32, [vp, P(i64)]),
         "anufft_debug_sort": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
         "anufft_debug_spread": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),

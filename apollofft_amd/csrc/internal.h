@@ -186,6 +186,14 @@ hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const 
 // Direct FP64 sums (direct.hip): type 1 writes every mode, type 2 every point.
 hipError_t launch_direct(int type, const PointArgs &pa, int dim, const int64_t n[3],
                          const double L[3], const float2 *in, double2 *out, hipStream_t s);
+// Type-1 at nsel selected flat mode indices, complex128 values, FP64.
+hipError_t launch_direct_select(const PointArgs &pa, int dim, const int64_t n[3],
+                                const double L[3], const double2 *values, const int64_t *sel,
+                                int64_t nsel, double2 *out, hipStream_t s);
+// The same sums over FP64 coordinates (stride as PointArgs) and complex128 data.
+hipError_t launch_direct_f64(int type, const double *x, const double *y, const double *z,
+                             int64_t stride, int64_t num_points, int dim, const int64_t n[3],
+                             const double L[3], const double2 *in, double2 *out, hipStream_t s);
 
 hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t num_points,
                            const float2 *values, float2 *grid, const Geom &g,

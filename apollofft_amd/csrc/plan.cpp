@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -158,10 +159,12 @@ std::vector<int> factorize(int L) {
 
 size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
-// Per-stage HIP-event timing (diagnostic; enabled per plan).
+// Per-stage HIP-event timing (diagnostic; enabled per plan).  The mutex
+// guards the event pool and records only; it is never held while a stage
+// runs, so it nests inside no other plan lock.
 struct StageTimer {
   std::mutex mu;
-  bool on = false;
+  std::atomic<bool> on{false};
   struct Rec {
     int stage;
     hipEvent_t a, b;
@@ -633,13 +636,22 @@ struct Exec {
   template <typename F>
   hipError_t timed(int stage, F f) const {
     StageTimer &t = p->timer;
-    if (!t.on) return f();
-    std::lock_guard<std::mutex> lk(t.mu);
-    hipEvent_t a = t.get(), b = t.get();
+    if (!t.on.load()) return f();
+    hipEvent_t a, b;
+    {
+      std::lock_guard<std::mutex> lk(t.mu);
+      a = t.get();
+      b = t.get();
+    }
     if (a) (void)hipEventRecord(a, s);
     hipError_t e = f();
     if (b) (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lk(t.mu);
     if (a && b) t.recs.push_back({stage, a, b});
+    else {
+      if (a) t.pool.push_back(a);
+      if (b) t.pool.push_back(b);
+    }
     return e;
   }
   // Run aux (an Exec on the plan's second stream) concurrently with main
@@ -782,11 +794,24 @@ struct Exec {
   }
 };
 
-anufft_status check_exec(const anufft_plan *p, int64_t M, const float *const coords[3],
-                         const void *a, const void *b, void *ws, size_t ws_bytes, Layout &lay) {
-  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+// Point counts: the subproblem list and the per-point records carry point
+// positions as signed 32-bit integers, so at most 2^31 - 1 points per call.
+constexpr int64_t kMaxPoints = 2147483647;
+
+anufft_status check_points(int64_t M, int64_t stride) {
   if (M < 0) return fail(ANUFFT_INVALID_ARG, "num_points must be >= 0");
-  if (M > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points %lld exceeds u32", (long long)M);
+  if (M > kMaxPoints)
+    return fail(ANUFFT_U32_OVERFLOW, "num_points %lld exceeds the limit 2^31 - 1", (long long)M);
+  if (stride < 1) return fail(ANUFFT_INVALID_ARG, "coordinate stride must be >= 1 (got %lld)", (long long)stride);
+  return ANUFFT_OK;
+}
+
+anufft_status check_exec(const anufft_plan *p, int64_t M, const float *const coords[3],
+                         int64_t stride, const void *a, const void *b, void *ws, size_t ws_bytes,
+                         Layout &lay) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  anufft_status sp = check_points(M, stride);
+  if (sp != ANUFFT_OK) return sp;
   if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
     return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
   if ((M > 0 && !a) || !b) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
@@ -802,11 +827,10 @@ anufft_status check_exec(const anufft_plan *p, int64_t M, const float *const coo
 anufft_status direct(int type, const anufft_plan *p, int64_t M, const float *const coords[3],
                      int64_t stride, const float *in, double *out, void *stream) {
   if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
-  if (M < 0) return fail(ANUFFT_INVALID_ARG, "num_points must be >= 0");
-  if (M > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points %lld exceeds u32", (long long)M);
+  anufft_status sp = check_points(M, stride);
+  if (sp != ANUFFT_OK) return sp;
   if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
     return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
-  if (stride < 1) return fail(ANUFFT_INVALID_ARG, "coordinate stride must be >= 1");
   if (M > 0 && !in) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
   if (!out && (type == 1 || M > 0)) return fail(ANUFFT_INVALID_ARG, "output pointer is NULL");
   if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
@@ -961,7 +985,7 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
                                    int64_t stride, const float *values, float *out, void *ws,
                                    size_t ws_bytes, void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, values, out, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, values, out, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
@@ -997,7 +1021,7 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
                                    int64_t stride, const float *modes, float *out, void *ws,
                                    size_t ws_bytes, void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, out, modes, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, out, modes, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   if (M == 0) return ANUFFT_OK;
   DeviceGuard guard(p->device);
@@ -1044,7 +1068,7 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
                                          float *const grad[3], void *ws, size_t ws_bytes,
                                          void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, u, modes_h, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, u, modes_h, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   if (!grad || !grad[0]) return fail(ANUFFT_INVALID_ARG, "grad[0] is NULL");
   if (M == 0) return ANUFFT_OK;
@@ -1086,7 +1110,7 @@ anufft_status anufft_execute_type2_backward(const anufft_plan *p, int64_t M,
                                             float *grad_modes, float *const grad_x[3], void *ws,
                                             size_t ws_bytes, void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, g, modes, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, g, modes, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   if (!grad_modes) return fail(ANUFFT_INVALID_ARG, "grad_modes is NULL");
   if (M > 0 && (!grad_x || !grad_x[0])) return fail(ANUFFT_INVALID_ARG, "grad_x[0] is NULL");
@@ -1134,7 +1158,7 @@ anufft_status anufft_execute_type1_backward(const anufft_plan *p, int64_t M,
                                             float *grad_values, float *const grad_x[3], void *ws,
                                             size_t ws_bytes, void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, values, G, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, values, G, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   if (M > 0 && !grad_values) return fail(ANUFFT_INVALID_ARG, "grad_values is NULL");
   if (M > 0 && (!grad_x || !grad_x[0])) return fail(ANUFFT_INVALID_ARG, "grad_x[0] is NULL");
@@ -1181,7 +1205,7 @@ anufft_status anufft_timing_enable(const anufft_plan *p, int32_t on) {
     p->timer.pool.push_back(r.b);
   }
   p->timer.recs.clear();
-  p->timer.on = on != 0;
+  p->timer.on.store(on != 0);
   return ANUFFT_OK;
 }
 
@@ -1220,6 +1244,32 @@ anufft_status anufft_direct_type2(const anufft_plan *p, int64_t M, const float *
   return direct(2, p, M, coords, stride, modes, out, stream);
 }
 
+anufft_status anufft_direct_type1_select(const anufft_plan *p, int64_t M,
+                                         const float *const coords[3], int64_t stride,
+                                         const double *values, int64_t num_modes,
+                                         const int64_t *mode_index, double *out, void *stream) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  anufft_status sp = check_points(M, stride);
+  if (sp != ANUFFT_OK) return sp;
+  if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
+    return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
+  if (num_modes < 0) return fail(ANUFFT_INVALID_ARG, "num_modes must be >= 0");
+  if ((M > 0 && !values) || (num_modes > 0 && (!mode_index || !out)))
+    return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
+  if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  DeviceGuard guard(p->device);
+  PointArgs pa{};
+  pa.x = M > 0 ? coords[0] : nullptr;
+  pa.y = M > 0 && p->dim == 3 ? coords[1] : pa.x;
+  pa.z = M > 0 && p->dim == 3 ? coords[2] : pa.x;
+  pa.stride = stride;
+  pa.num_points = M;
+  hipError_t e = launch_direct_select(pa, p->dim, p->n, p->L, (const double2 *)values, mode_index,
+                                      num_modes, (double2 *)out, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "direct sum (selected modes)");
+  return ANUFFT_OK;
+}
+
 anufft_status anufft_debug_num_bins(const anufft_plan *p, int64_t *num_bins) {
   if (!p || !num_bins) return fail(ANUFFT_INVALID_ARG, "NULL argument");
   *num_bins = p->bs.nbins;
@@ -1230,7 +1280,7 @@ anufft_status anufft_debug_sort(const anufft_plan *p, int64_t M, const float *co
                                 int64_t stride, uint32_t *perm, uint32_t *bin_start, void *ws,
                                 size_t ws_bytes, void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, perm, bin_start, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, perm, bin_start, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   if (p->dim != 3) return fail(ANUFFT_UNSUPPORTED, "1D plans do not bin points");
   DeviceGuard guard(p->device);
@@ -1250,7 +1300,7 @@ anufft_status anufft_debug_spread(const anufft_plan *p, int64_t M, const float *
                                   int64_t stride, const float *values, float *grid, void *ws,
                                   size_t ws_bytes, void *stream) {
   Layout lay;
-  anufft_status st = check_exec(p, M, coords, values, grid, ws, ws_bytes, lay);
+  anufft_status st = check_exec(p, M, coords, stride, values, grid, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
@@ -1304,11 +1354,12 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   if (!coords || !values || !out) return fail(ANUFFT_INVALID_ARG, "NULL argument");
   bool fused = true;
   for (int i = 0; i < nprob; ++i) {
+    const anufft_status sp = check_points(num_points[i], stride);
+    if (sp != ANUFFT_OK) return fail(sp, "problem %d: %s", i, std::string(g_last_error).c_str());
     if (plans[i]->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for plan %d", i);
     if (plans[i]->device != plans[0]->device) return fail(ANUFFT_INVALID_ARG, "plans live on different devices");
     if (!out[i] || (num_points[i] > 0 && (!values[i] || !coords[3 * i] || !coords[3 * i + 1] || !coords[3 * i + 2])))
       return fail(ANUFFT_INVALID_ARG, "NULL pointer in problem %d", i);
-    if (num_points[i] > (int64_t)UINT32_MAX) return fail(ANUFFT_U32_OVERFLOW, "num_points[%d] exceeds u32", i);
     if (plans[i]->w != plans[0]->w || plans[i]->sigma != plans[0]->sigma ||
         plans[i]->kernel_method != plans[0]->kernel_method)
       fused = false;
@@ -1507,7 +1558,8 @@ anufft_status host_exec(const anufft_plan *p, int type, int32_t storage, int32_t
     return fail(ANUFFT_PROFILE_MISMATCH,
                 "invalid field precision_profile: profile %d does not match storage %d", profile,
                 storage);
-  if (M < 0) return fail(ANUFFT_INVALID_ARG, "num_points must be >= 0");
+  anufft_status sp = check_points(M, stride);
+  if (sp != ANUFFT_OK) return sp;
   if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
     return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
   if ((M > 0 || type == 1) && (!in || !out)) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
@@ -1550,9 +1602,62 @@ anufft_status host_exec(const anufft_plan *p, int type, int32_t storage, int32_t
   return ANUFFT_OK;
 }
 
+// Exact direct sums from host pointers: FP64 coordinates and complex128
+// data copied to the device unchanged, FP64 arithmetic, complex128 out.
+anufft_status host_direct(const anufft_plan *p, int type, int64_t M, const double *const coords[3],
+                          int64_t stride, const double *in, double *out) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  anufft_status sp = check_points(M, stride);
+  if (sp != ANUFFT_OK) return sp;
+  if (M > 0 && (!coords || !coords[0] || (p->dim == 3 && (!coords[1] || !coords[2]))))
+    return fail(ANUFFT_INVALID_ARG, "coordinate pointer is NULL");
+  if ((M > 0 || type == 1) && (!in || !out)) return fail(ANUFFT_INVALID_ARG, "data pointer is NULL");
+  if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  if (type == 2 && M == 0) return ANUFFT_OK;
+  DeviceGuard guard(p->device);
+  hipStream_t s = p->host_stream;
+  const int64_t N = p->modes();
+  const int64_t nin = type == 1 ? M : N, nout = type == 1 ? N : M;
+  std::vector<double> hc((size_t)std::max<int64_t>(M, 1) * p->dim);
+  for (int a = 0; a < p->dim; ++a)
+    for (int64_t j = 0; j < M; ++j) hc[(size_t)a * M + j] = coords[a][j * stride];
+  size_t b_in = align_up(16 * (size_t)std::max<int64_t>(nin, 1));
+  size_t b_out = align_up(16 * (size_t)std::max<int64_t>(nout, 1));
+  size_t b_c = align_up(8 * hc.size());
+  char *buf = nullptr;
+  hipError_t e = hipMalloc(&buf, b_in + b_out + b_c);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  double2 *d_in = (double2 *)buf, *d_out = (double2 *)(buf + b_in);
+  double *d_c = (double *)(buf + b_in + b_out);
+  do {
+    if (nin > 0 && (e = hipMemcpyAsync(d_in, in, 16 * (size_t)nin, hipMemcpyHostToDevice, s)) != hipSuccess) break;
+    if ((e = hipMemcpyAsync(d_c, hc.data(), 8 * hc.size(), hipMemcpyHostToDevice, s)) != hipSuccess) break;
+    const double *dx = d_c, *dy = p->dim == 3 ? d_c + M : d_c, *dz = p->dim == 3 ? d_c + 2 * M : d_c;
+    if ((e = launch_direct_f64(type, dx, dy, dz, 1, M, p->dim, p->n, p->L, d_in, d_out, s)) != hipSuccess) break;
+    if ((e = hipMemcpyAsync(out, d_out, 16 * (size_t)nout, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+    e = hipStreamSynchronize(s);
+  } while (0);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return hip_fail(e, "direct sum (host)");
+  return ANUFFT_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+anufft_status anufft_direct_type1_host(const anufft_plan *p, int64_t M,
+                                       const double *const coords[3], int64_t stride,
+                                       const double *values, double *out) {
+  return host_direct(p, 1, M, coords, stride, values, out);
+}
+
+anufft_status anufft_direct_type2_host(const anufft_plan *p, int64_t M,
+                                       const double *const coords[3], int64_t stride,
+                                       const double *modes, double *out) {
+  return host_direct(p, 2, M, coords, stride, modes, out);
+}
 
 anufft_status anufft_execute_type1_host(const anufft_plan *p, int32_t storage, int32_t profile,
                                         int64_t M, const double *const coords[3], int64_t stride,

@@ -44,6 +44,24 @@ def _policy(p):
     raise ValueError(f"unknown grid_policy {p!r}")
 
 
+def _check_tensor(t, name, dtype, device, shape=None, numel=None):
+    """Raise ValueError unless t is a contiguous tensor of `dtype` on `device`
+    with the given shape or element count (the kernels take raw pointers)."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype} (got {t.dtype})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, the positions on {device}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)} (got {tuple(t.shape)})")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} must have {numel} elements (got {t.numel()})")
+
+
 class _PlanBase:
     dim = 0
 
@@ -134,11 +152,15 @@ class _PlanBase:
             M = cs[0].numel()
             if any(c.numel() != M for c in cs):
                 raise ValueError("coordinate arrays differ in length")
+            if any(c.device != cs[0].device for c in cs) or cs[0].device.type != "cuda":
+                raise ValueError("coordinate arrays must be on one GPU")
             ptrs = [c.data_ptr() for c in cs] + [cs[0].data_ptr()] * (3 - len(cs))
             return ptrs, 1, M, cs[0].device
         t = positions
-        if t.dtype != torch.float32 or not t.is_contiguous():
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_contiguous():
             raise ValueError("positions must be a contiguous float32 tensor")
+        if t.device.type != "cuda":
+            raise ValueError("positions must be on a GPU")
         if self.dim == 1:
             if t.dim() != 1:
                 raise ValueError("1D positions must have shape (M,)")
@@ -157,13 +179,12 @@ class _PlanBase:
         """Type-1: non-uniform samples -> modes (dimension_3d.rs:256-330)."""
         import torch
         ptrs, stride, M, dev = self._coords(positions)
-        if values.dtype != torch.complex64 or not values.is_contiguous() or values.numel() != M:
-            raise ValueError("values must be a contiguous complex64 tensor with one entry per point")
+        _check_tensor(values, "values", torch.complex64, dev, numel=M)
         shape = self.modes_shape
         if out is None:
             out = torch.empty(shape, dtype=torch.complex64, device=dev)
-        elif tuple(out.shape) != shape or out.dtype != torch.complex64 or not out.is_contiguous():
-            raise ValueError(f"out must be a contiguous complex64 tensor of shape {shape}")
+        else:
+            _check_tensor(out, "out", torch.complex64, dev, shape=shape)
         ws = self.workspace(M, dev)
         N.check(N.lib().anufft_execute_type1(
             self._h, M, N.ptr_array(ptrs), stride, values.data_ptr(), out.data_ptr(),
@@ -175,10 +196,11 @@ class _PlanBase:
         import torch
         ptrs, stride, M, dev = self._coords(positions)
         shape = self.modes_shape
-        if modes.dtype != torch.complex64 or not modes.is_contiguous() or tuple(modes.shape) != shape:
-            raise ValueError(f"modes must be a contiguous complex64 tensor of shape {shape}")
+        _check_tensor(modes, "modes", torch.complex64, dev, shape=shape)
         if out is None:
             out = torch.empty(M, dtype=torch.complex64, device=dev)
+        else:
+            _check_tensor(out, "out", torch.complex64, dev, numel=M)
         ws = self.workspace(M, dev)
         N.check(N.lib().anufft_execute_type2(
             self._h, M, N.ptr_array(ptrs), stride, modes.data_ptr(), out.data_ptr(),
@@ -190,12 +212,29 @@ class _PlanBase:
         WGPU direct backend, nufft.wgsl:92-117, computed there in f32)."""
         import torch
         ptrs, stride, M, dev = self._coords(positions)
-        if values.dtype != torch.complex64 or not values.is_contiguous() or values.numel() != M:
-            raise ValueError("values must be a contiguous complex64 tensor with one entry per point")
+        _check_tensor(values, "values", torch.complex64, dev, numel=M)
         out = torch.empty(self.modes_shape, dtype=torch.complex128, device=dev)
         N.check(N.lib().anufft_direct_type1(self._h, M, N.ptr_array(ptrs), stride,
                                             values.data_ptr(), out.data_ptr(), self._stream(dev)),
                 "direct_type1")
+        return out
+
+    def direct_type1_select(self, positions, values, mode_index):
+        """Exact Type-1 sum in FP64 at chosen flat mode indices (int64 tensor)
+        -> complex128, one entry per index (anufft_direct_type1_select)."""
+        import torch
+        ptrs, stride, M, dev = self._coords(positions)
+        if values.dtype == torch.complex64:
+            values = values.to(torch.complex128)
+        _check_tensor(values, "values", torch.complex128, dev, numel=M)
+        _check_tensor(mode_index, "mode_index", torch.int64, dev)
+        nmodes = math.prod(self.modes_shape)
+        if mode_index.numel() and (int(mode_index.min()) < 0 or int(mode_index.max()) >= nmodes):
+            raise ValueError(f"mode_index must lie in [0, {nmodes})")
+        out = torch.empty(mode_index.numel(), dtype=torch.complex128, device=dev)
+        N.check(N.lib().anufft_direct_type1_select(
+            self._h, M, N.ptr_array(ptrs), stride, values.data_ptr(), mode_index.numel(),
+            mode_index.data_ptr(), out.data_ptr(), self._stream(dev)), "direct_type1_select")
         return out
 
     def direct_type2(self, positions, modes):
@@ -203,9 +242,7 @@ class _PlanBase:
         (the WGPU direct backend, nufft.wgsl:119-140)."""
         import torch
         ptrs, stride, M, dev = self._coords(positions)
-        shape = self.modes_shape
-        if modes.dtype != torch.complex64 or not modes.is_contiguous() or tuple(modes.shape) != shape:
-            raise ValueError(f"modes must be a contiguous complex64 tensor of shape {shape}")
+        _check_tensor(modes, "modes", torch.complex64, dev, shape=self.modes_shape)
         out = torch.empty(M, dtype=torch.complex128, device=dev)
         N.check(N.lib().anufft_direct_type2(self._h, M, N.ptr_array(ptrs), stride,
                                             modes.data_ptr(), out.data_ptr(), self._stream(dev)),
@@ -216,10 +253,8 @@ class _PlanBase:
         """grad[a][j] = Re(conj(u_j) d/dx_j^(a) sum_k h_k e^{+i k.x_j}) -> (M, dim)."""
         import torch
         ptrs, stride, M, dev = self._coords(positions)
-        if u.dtype != torch.complex64 or not u.is_contiguous() or u.numel() != M:
-            raise ValueError("u must be a contiguous complex64 tensor with one entry per point")
-        if modes_h.dtype != torch.complex64 or not modes_h.is_contiguous() or tuple(modes_h.shape) != self.modes_shape:
-            raise ValueError("modes_h must be a contiguous complex64 tensor of the mode shape")
+        _check_tensor(u, "u", torch.complex64, dev, numel=M)
+        _check_tensor(modes_h, "modes_h", torch.complex64, dev, shape=self.modes_shape)
         g = torch.empty((self.dim, M), dtype=torch.float32, device=dev)
         gp = [g[a].data_ptr() for a in range(self.dim)] + [None] * (3 - self.dim)
         ws = self.workspace(M, dev)
@@ -241,10 +276,8 @@ class _PlanBase:
         binning of the points (anufft_execute_type2_backward)."""
         import torch
         ptrs, stride, M, dev = self._coords(positions)
-        if g.dtype != torch.complex64 or not g.is_contiguous() or g.numel() != M:
-            raise ValueError("g must be a contiguous complex64 tensor with one entry per point")
-        if modes.dtype != torch.complex64 or not modes.is_contiguous() or tuple(modes.shape) != self.modes_shape:
-            raise ValueError("modes must be a contiguous complex64 tensor of the mode shape")
+        _check_tensor(g, "g", torch.complex64, dev, numel=M)
+        _check_tensor(modes, "modes", torch.complex64, dev, shape=self.modes_shape)
         gm = torch.empty(self.modes_shape, dtype=torch.complex64, device=dev)
         gx = torch.empty((self.dim, M), dtype=torch.float32, device=dev)
         gp = [gx[a].data_ptr() for a in range(self.dim)] + [None] * (3 - self.dim)
@@ -260,10 +293,8 @@ class _PlanBase:
         G and one binning (anufft_execute_type1_backward)."""
         import torch
         ptrs, stride, M, dev = self._coords(positions)
-        if values.dtype != torch.complex64 or not values.is_contiguous() or values.numel() != M:
-            raise ValueError("values must be a contiguous complex64 tensor with one entry per point")
-        if G.dtype != torch.complex64 or not G.is_contiguous() or tuple(G.shape) != self.modes_shape:
-            raise ValueError("G must be a contiguous complex64 tensor of the mode shape")
+        _check_tensor(values, "values", torch.complex64, dev, numel=M)
+        _check_tensor(G, "G", torch.complex64, dev, shape=self.modes_shape)
         gv = torch.empty(M, dtype=torch.complex64, device=dev)
         gx = torch.empty((self.dim, M), dtype=torch.float32, device=dev)
         gp = [gx[a].data_ptr() for a in range(self.dim)] + [None] * (3 - self.dim)
@@ -285,6 +316,34 @@ class _PlanBase:
         pos = np.ascontiguousarray(pos.reshape(-1, 3))
         base = pos.ctypes.data
         return pos, [base, base + 8, base + 16], 3, pos.shape[0]
+
+    def direct_type1_host(self, positions, values):
+        """Exact Type-1 sum, FP64 coordinates and arithmetic (anufft_direct_type1_host):
+        numpy float64 positions, complex128 values -> complex128 modes."""
+        import numpy as np
+        keep, ptrs, stride, M = self._host_coords(positions)
+        vals = np.ascontiguousarray(values, dtype=np.complex128).reshape(-1)
+        if vals.size != M:
+            raise N.ShapeMismatch(f"positions/value length mismatch: {M} vs {vals.size}")
+        out = np.empty(self.modes_shape, dtype=np.complex128)
+        N.check(N.lib().anufft_direct_type1_host(self._h, M, N.ptr_array(ptrs), stride,
+                                                 vals.ctypes.data, out.ctypes.data),
+                "direct_type1_host")
+        return out
+
+    def direct_type2_host(self, positions, modes):
+        """Exact Type-2 sum, FP64 coordinates and arithmetic (anufft_direct_type2_host):
+        complex128 modes of the plan's shape -> complex128 per point."""
+        import numpy as np
+        keep, ptrs, stride, M = self._host_coords(positions)
+        md = np.ascontiguousarray(modes, dtype=np.complex128)
+        if md.size != math.prod(self.modes_shape):
+            raise N.ShapeMismatch(f"modes shape mismatch: expected {self.modes_shape}")
+        out = np.empty(M, dtype=np.complex128)
+        N.check(N.lib().anufft_direct_type2_host(self._h, M, N.ptr_array(ptrs), stride,
+                                                 md.ctypes.data, out.ctypes.data),
+                "direct_type2_host")
+        return out
 
     def type1_host(self, positions, values, storage=N.STORAGE_C128, profile=None):
         import numpy as np
@@ -366,15 +425,28 @@ def execute_type1_batched(plans, positions, values, outs=None):
     if not (len(positions) == len(values) == n):
         raise ValueError("plans, positions and values differ in length")
     coords, vals, Ms, dev = [], [], [], None
-    for p, pos, v in zip(plans, positions, values):
-        ptrs, stride, M, dev = p._coords(pos)
-        if stride != 3 and n > 0:
+    for i, (p, pos, v) in enumerate(zip(plans, positions, values)):
+        if p.dim != 3:
+            raise ValueError("batched execution takes 3D plans")
+        ptrs, stride, M, d = p._coords(pos)
+        if stride != 3:
             raise ValueError("batched execution takes (M, 3) position tensors")
+        if dev is not None and d != dev:
+            raise ValueError("all problems of a batch must be on one GPU")
+        dev = d
+        _check_tensor(v, f"values[{i}]", torch.complex64, dev, numel=M)
         coords += ptrs
         vals.append(v.data_ptr())
         Ms.append(M)
+    if n == 0:
+        return [] if outs is None else outs
     if outs is None:
         outs = [torch.empty(p.modes_shape, dtype=torch.complex64, device=dev) for p in plans]
+    elif len(outs) != n:
+        raise ValueError("outs and plans differ in length")
+    else:
+        for i, (p, o) in enumerate(zip(plans, outs)):
+            _check_tensor(o, f"outs[{i}]", torch.complex64, dev, shape=p.modes_shape)
     hs = N.ptr_array([p.handle.value for p in plans])
     Marr = (ctypes.c_int64 * n)(*Ms)
     b = ctypes.c_size_t()

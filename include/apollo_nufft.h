@@ -130,8 +130,10 @@ anufft_status anufft_workspace_size(const anufft_plan *plan, int64_t max_points,
  * (dimension_1d.rs:166, dimension_3d.rs:256) and NufftWgpuBackend::
  * execute_fast_type1_{1d,3d}[_with_buffers] (device.rs:217,383,681,752).
  * Point coordinate j of axis a is read at coords[a][j * coord_stride]
- * (SoA: stride 1; AoS (x,y,z) tuples: stride 3).  1D plans read coords[0]
- * only.  values: M complex; out: n (1D) or nx*ny*nz (3D) complex. */
+ * (SoA: stride 1; AoS (x,y,z) tuples: stride 3; a stride below 1 is
+ * ANUFFT_INVALID_ARG).  1D plans read coords[0] only.  values: M complex;
+ * out: n (1D) or nx*ny*nz (3D) complex.  Every execute call takes at most
+ * 2^31 - 1 points (more: ANUFFT_U32_OVERFLOW). */
 anufft_status anufft_execute_type1(const anufft_plan *plan, int64_t num_points,
                                    const float *const coords[3], int64_t coord_stride,
                                    const float *values, float *out, void *workspace,
@@ -197,9 +199,10 @@ anufft_status anufft_execute_type1_backward(const anufft_plan *plan, int64_t num
 /* Batched Type-1 over independent 3D plans (no reference counterpart: the
  * reference has no batched NUFFT).  Problem p uses plans[p], num_points[p],
  * coords_p[a] = coords[3p + a] (stride coord_stride), values[p], out[p].  The
- * workspace must hold anufft_batch_workspace_size bytes.  All problems run in
- * one stream; binning and spreading of every problem are fused into shared
- * launches. */
+ * workspace must hold anufft_batch_workspace_size bytes.  Work is queued on
+ * `hip_stream`; parts of it are forked to the second stream of plans[0] and
+ * joined back.  Problems with equal kernel width and sigma share one
+ * persistent spreading launch; the binning and FFT passes are per problem. */
 anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32_t num_problems,
                                           const int64_t *max_points, size_t *bytes);
 anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int32_t num_problems,
@@ -261,6 +264,32 @@ anufft_status anufft_direct_type1(const anufft_plan *plan, int64_t num_points,
 anufft_status anufft_direct_type2(const anufft_plan *plan, int64_t num_points,
                                   const float *const coords[3], int64_t coord_stride,
                                   const float *modes, double *out, void *stream);
+
+/* Exact Type-1 sum at chosen modes only: out[s] = sum_j c_j exp(-2 pi i
+ * k(mode_index[s]).x_j / L), mode_index[s] a flat FFT-order index into
+ * [nx][ny][nz] (1D: [n]).  Device pointers: float32 coordinates, complex128
+ * values, int64 indices, complex128 out; FP64 arithmetic.  An extension of
+ * the direct backend above (which always writes every mode) for sampled
+ * accuracy checks of large problems; no workspace, queued on `stream`. */
+anufft_status anufft_direct_type1_select(const anufft_plan *plan, int64_t num_points,
+                                         const float *const coords[3], int64_t coord_stride,
+                                         const double *values, int64_t num_modes,
+                                         const int64_t *mode_index, double *out, void *stream);
+
+/* The same exact sums from host pointers: FP64 coordinates (stride as for
+ * execute) and complex128 values / modes, copied to the device unchanged;
+ * FP64 arithmetic; complex128 out (host).  Replaces the CPU direct sums
+ * nufft_type1_1d, nufft_type2_1d (apollo-nufft .../transform/
+ * dimension_1d.rs:429-472) and nufft_type1_3d, nufft_type2_3d
+ * (dimension_3d.rs:679-740) behind pyapollofft.nufft_type1_1d,
+ * nufft_type2_1d and nufft_type1_3d (apollo-python/src/lib.rs:754-836).
+ * Allocates device memory per call and synchronizes. */
+anufft_status anufft_direct_type1_host(const anufft_plan *plan, int64_t num_points,
+                                       const double *const coords[3], int64_t coord_stride,
+                                       const double *values, double *out);
+anufft_status anufft_direct_type2_host(const anufft_plan *plan, int64_t num_points,
+                                       const double *const coords[3], int64_t coord_stride,
+                                       const double *modes, double *out);
 
 /* Per-stage timing with HIP events recorded on the execution stream
  * (no reference counterpart; the reference records no timings, every WGPU

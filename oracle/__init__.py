@@ -60,6 +60,7 @@ def lib():
             "or_direct_type2_3d": (None, [vp, vp, vp, i64, vp, vp]),
             "or_direct_type2_3d_grad_x": (None, [vp, vp, vp, i64, vp, vp, vp]),
             "or_num_threads": (i, []),
+            "or_set_fast_threads": (None, [i]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -249,3 +250,9 @@ def direct_type2_3d_grad_x(f, xyz, d, r):
 
 def num_threads():
     return lib().or_num_threads()
+
+
+def set_fast_threads(n):
+    """Threads of the 3D fast restatement (1: the reference's single-thread
+    path; more: the multi-thread CPU baseline column)."""
+    lib().or_set_fast_threads(int(n))

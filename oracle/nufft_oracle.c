@@ -216,6 +216,14 @@ void or_fft_inplace(double *data, int64_t n, int sign) {
   fft_plan_free(&pl);
 }
 
+/* Threads of the 3D fast restatement (1 = the reference's single-threaded
+ * path).  With more, the CPU baseline's multi-thread column runs the same
+ * algorithm split over points (spreading into per-thread grids, then a
+ * reduction), FFT lanes and interpolated points. */
+static int g_fast_threads = 1;
+
+void or_set_fast_threads(int n) { g_fast_threads = n > 1 ? n : 1; }
+
 /* Separable 3D FFT over a [mx][my][mz] grid, axis order given.
  * Forward: z, y, x (dimension_3d.rs:396-398, fft_{z,y,x}_pass :631-674).
  * Inverse: x, y, z (dimension_3d.rs:549-551, ifft_{x,y,z}_pass :416-459),
@@ -235,6 +243,28 @@ static void grid_fft3(cpx *g, const int64_t m[3], int sign, int normalize) {
     double sc = normalize ? 1.0 / (double)len : 1.0;
     int64_t stride = ax == 0 ? my * mz : (ax == 1 ? mz : 1);
     int64_t na = ax == 0 ? my : mx, nb = ax == 2 ? my : mz;
+    if (g_fast_threads > 1) {
+#pragma omp parallel num_threads(g_fast_threads)
+      {
+        fft_plan tp;
+        fft_plan_init(&tp, len, sign);
+        cpx *tl = (cpx *)malloc(sizeof(cpx) * (size_t)lmax);
+#pragma omp for schedule(static)
+        for (int64_t ab = 0; ab < na * nb; ++ab) {
+          int64_t a = ab / nb, b = ab % nb, base;
+          if (ax == 0) base = a * mz + b;
+          else if (ax == 1) base = a * my * mz + b;
+          else base = a * my * mz + b * mz;
+          for (int64_t i = 0; i < len; ++i) tl[i] = g[base + i * stride];
+          fft_exec(&tp, tl);
+          for (int64_t i = 0; i < len; ++i) g[base + i * stride] = cscale(tl[i], sc);
+        }
+        free(tl);
+        fft_plan_free(&tp);
+      }
+      fft_plan_free(&pl);
+      continue;
+    }
     for (int64_t a = 0; a < na; ++a)
       for (int64_t b = 0; b < nb; ++b) {
         int64_t base;
@@ -450,20 +480,14 @@ static void point_weights(const plan3 *pl, const pt3 *p, int64_t m0[3], double *
   }
 }
 
-/* Spread onto the oversampled grid (T1 hot loop, dimension_3d.rs:277-327).
- * Exposed so tests can pin the device spreader on the fine grid itself. */
-int or_spread3d(const int64_t n[3], const double d[3], int64_t sigma, int64_t w, int policy,
-                int64_t M, const double *xyz, const double *c, double *grid_out) {
-  plan3 pl;
-  if (plan3_init(&pl, n, d, sigma, w, policy)) return 1;
-  int64_t mx = pl.m[0], my = pl.m[1], mz = pl.m[2];
-  cpx *grid = (cpx *)grid_out;
-  memset(grid, 0, sizeof(cpx) * (size_t)(mx * my * mz));
-  pt3 *pts = sort_points_3d(xyz, c, M, pl.L, pl.m, w);
+/* The T1 hot loop over points (dimension_3d.rs:290-327): per point the
+ * (2w+1)^3 taps with zero-skip per axis, added into grid. */
+static void spread_points(const plan3 *pl, const pt3 *pts, int64_t M, cpx *grid) {
+  const int64_t mx = pl->m[0], my = pl->m[1], mz = pl->m[2], w = pl->w;
   double wx[64], wy[64], wz[64];
   int64_t m0[3];
   for (int64_t j = 0; j < M; ++j) {
-    point_weights(&pl, &pts[j], m0, wx, wy, wz);
+    point_weights(pl, &pts[j], m0, wx, wy, wz);
     for (int64_t px = 0; px <= 2 * w; ++px) {
       if (wx[px] == 0.0) continue;
       int64_t ix = or_irem(m0[0] + px - w, mx);
@@ -479,6 +503,42 @@ int or_spread3d(const int64_t n[3], const double d[3], int64_t sigma, int64_t w,
         }
       }
     }
+  }
+}
+
+/* Spread onto the oversampled grid (T1 hot loop, dimension_3d.rs:277-327).
+ * Exposed so tests can pin the device spreader on the fine grid itself. */
+int or_spread3d(const int64_t n[3], const double d[3], int64_t sigma, int64_t w, int policy,
+                int64_t M, const double *xyz, const double *c, double *grid_out) {
+  plan3 pl;
+  if (plan3_init(&pl, n, d, sigma, w, policy)) return 1;
+  int64_t mx = pl.m[0], my = pl.m[1], mz = pl.m[2];
+  cpx *grid = (cpx *)grid_out;
+  memset(grid, 0, sizeof(cpx) * (size_t)(mx * my * mz));
+  pt3 *pts = sort_points_3d(xyz, c, M, pl.L, pl.m, w);
+  if (g_fast_threads > 1) {
+    /* contiguous runs of the sorted points into per-thread grids, then a
+     * reduction over cells (multi-thread CPU baseline only) */
+    const int64_t G = mx * my * mz;
+    const int T = g_fast_threads;
+    cpx **priv = (cpx **)malloc(sizeof(cpx *) * (size_t)T);
+#pragma omp parallel num_threads(T)
+    {
+      const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+      cpx *mine = t == 0 ? grid : (cpx *)calloc((size_t)G, sizeof(cpx));
+      priv[t] = mine;
+      const int64_t j0 = M * t / nt, j1 = M * (t + 1) / nt;
+      spread_points(&pl, pts + j0, j1 - j0, mine);
+#pragma omp barrier
+#pragma omp for schedule(static)
+      for (int64_t i = 0; i < G; ++i)
+        for (int u = 1; u < nt; ++u) grid[i] = cadd(grid[i], priv[u][i]);
+#pragma omp barrier
+      if (t > 0) free(mine);
+    }
+    free(priv);
+  } else {
+    spread_points(&pl, pts, M, grid);
   }
   free(pts);
   plan3_free(&pl);
@@ -539,10 +599,11 @@ int or_nufft3d_type2_fast(const int64_t n[3], const double d[3], int64_t sigma, 
     for (int64_t i = 0; i < mx * my * mz; ++i) grid[i] = cscale(grid[i], g);
   }
   pt3 *pts = sort_points_3d(xyz, NULL, M, pl.L, pl.m, w);
-  double wx[64], wy[64], wz[64];
-  int64_t m0[3];
   cpx *o = (cpx *)out;
+#pragma omp parallel for schedule(static) num_threads(g_fast_threads) if (g_fast_threads > 1)
   for (int64_t j = 0; j < M; ++j) {
+    double wx[64], wy[64], wz[64];
+    int64_t m0[3];
     point_weights(&pl, &pts[j], m0, wx, wy, wz);
     cpx acc = {0.0, 0.0};
     for (int64_t px = 0; px <= 2 * w; ++px) {
@@ -606,11 +667,26 @@ void or_direct_type2_1d(int64_t n, double dx, const double *f, int64_t M, const 
 
 /* Powers e^{i s k theta} for k_signed in [-(n-1)/2.., n/2], stored by FFT index. */
 static void axis_phases(double theta, int64_t n, cpx *tab) {
-  /* tab[k] = exp(i * k_signed(k) * theta) computed by direct sincos of the
-   * exact multiple (no recurrence), for accuracy at 1e-15. */
-  for (int64_t k = 0; k < n; ++k) {
-    double a = (double)or_fft_signed_index(k, n) * theta;
-    tab[k] = (cpx){cos(a), sin(a)};
+  /* tab[k] = exp(i * k_signed(k) * theta).  The non-negative powers s =
+   * 0..n/2 come from a direct sincos of the exact multiple every 8 steps and
+   * one complex multiplication by e^{i theta} in between (at most 7 rounding
+   * steps, ~1e-15 relative); negative indices take the conjugate of the
+   * matching positive power (theta is real). */
+  const cpx step = {cos(theta), sin(theta)};
+  const int64_t half = n / 2;
+  cpx cur = {1.0, 0.0};
+  for (int64_t s = 0; s <= half; ++s) {
+    if ((s & 7) == 0) {
+      const double a = (double)s * theta;
+      cur = (cpx){cos(a), sin(a)};
+    } else {
+      cur = cmul(cur, step);
+    }
+    tab[s] = cur;
+  }
+  for (int64_t k = half + 1; k < n; ++k) {
+    const cpx p = tab[n - k];
+    tab[k] = (cpx){p.re, -p.im};
   }
 }
 

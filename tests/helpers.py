@@ -39,3 +39,15 @@ def clustered_points(rng, M, nclusters=8, sigma=0.1, frac=0.7):
 
 def cnormal(rng, shape, dtype=np.complex64):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+
+
+def radial_points(rng, M, per_line=512):
+    """Points on ceil(M / per_line) lines through the origin, directions
+    uniform on the sphere, radius uniform in [-pi, pi) along each line: a 3D
+    density proportional to 1/r^2.  float32, shape (M, 3)."""
+    lines = -(-M // per_line)
+    dirs = rng.standard_normal((lines, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    r = rng.uniform(-np.pi, np.pi, size=(lines, per_line))
+    pts = (r[:, :, None] * dirs[:, None, :]).reshape(-1, 3)[:M]
+    return np.ascontiguousarray(pts, dtype=np.float32)

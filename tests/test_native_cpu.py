@@ -175,3 +175,63 @@ def test_validation_metric_is_the_reference_one():
     assert relative_complex_error(e, f) == pytest.approx(0.15)
     assert relative_complex_error(e, e) == 0.0
     assert relative_complex_error([], []) == 0.0
+
+
+def test_point_limit_and_stride_errors():
+    """More than 2^31 - 1 points is ANUFFT_U32_OVERFLOW and a coordinate
+    stride below 1 is ANUFFT_INVALID_ARG, for every entry point that takes
+    points, before any device work (no GPU needed)."""
+    lib = N.lib()
+    p = A.NufftPlan3D(8, 8, 8, 0.1, 0.1, 0.1)
+    buf = np.zeros(64, np.float32)
+    ptrs = N.ptr_array([buf.ctypes.data] * 3)
+    b = buf.ctypes.data
+    big = 2 ** 31
+    assert lib.anufft_execute_type1(p.handle, big, ptrs, 1, b, b, b, 1 << 40, None) == N.U32_OVERFLOW
+    assert lib.anufft_execute_type2(p.handle, big, ptrs, 1, b, b, b, 1 << 40, None) == N.U32_OVERFLOW
+    assert lib.anufft_execute_grad_points(p.handle, big, ptrs, 1, b, b, N.ptr_array([b] * 3), b,
+                                          1 << 40, None) == N.U32_OVERFLOW
+    assert lib.anufft_direct_type1(p.handle, big, ptrs, 1, b, b, None) == N.U32_OVERFLOW
+    for stride in (0, -3):
+        assert lib.anufft_execute_type1(p.handle, 3, ptrs, stride, b, b, b, 1 << 40, None) == N.INVALID_ARG
+        assert "stride" in lib.anufft_last_error().decode()
+        assert lib.anufft_execute_type2(p.handle, 3, ptrs, stride, b, b, b, 1 << 40, None) == N.INVALID_ARG
+        assert lib.anufft_execute_type2_backward(p.handle, 3, ptrs, stride, b, b, b,
+                                                 N.ptr_array([b] * 3), b, 1 << 40, None) == N.INVALID_ARG
+        assert lib.anufft_execute_type1_backward(p.handle, 3, ptrs, stride, b, b, b,
+                                                 N.ptr_array([b] * 3), b, 1 << 40, None) == N.INVALID_ARG
+        dptrs = N.ptr_array([np.zeros(8).ctypes.data] * 3)
+        assert lib.anufft_execute_type1_host(p.handle, 0, 0, 3, dptrs, stride, b, b) == N.INVALID_ARG
+        assert lib.anufft_direct_type1_host(p.handle, 3, dptrs, stride, b, b) == N.INVALID_ARG
+        hs = N.ptr_array([p.handle.value])
+        Ms = (ctypes.c_int64 * 1)(3)
+        assert lib.anufft_execute_type1_batched(hs, 1, Ms, ptrs, stride, N.ptr_array([b]),
+                                                N.ptr_array([b]), b, 1 << 40, None) == N.INVALID_ARG
+    hs = N.ptr_array([p.handle.value])
+    Ms = (ctypes.c_int64 * 1)(big)
+    assert lib.anufft_execute_type1_batched(hs, 1, Ms, ptrs, 3, N.ptr_array([b]), N.ptr_array([b]),
+                                            b, 1 << 62, None) == N.U32_OVERFLOW
+
+
+def test_python_wrappers_reject_bad_tensors():
+    """Tensor checks of the torch wrappers (dtype, contiguity, shape, device)
+    run before any pointer reaches the library."""
+    torch = pytest.importorskip("torch")
+    p = A.NufftPlan3D(4, 4, 4, 0.1, 0.1, 0.1)
+    pos = torch.zeros((5, 3), dtype=torch.float32)
+    with pytest.raises(ValueError, match="GPU"):
+        p.type1(pos, torch.zeros(5, dtype=torch.complex64))
+    from apollofft_amd.plan import _check_tensor
+    dev = torch.device("cpu")
+    with pytest.raises(ValueError, match="complex64"):
+        _check_tensor(torch.zeros(5), "values", torch.complex64, dev, numel=5)
+    with pytest.raises(ValueError, match="contiguous"):
+        _check_tensor(torch.zeros(10, dtype=torch.complex64)[::2], "values", torch.complex64, dev)
+    with pytest.raises(ValueError, match="elements"):
+        _check_tensor(torch.zeros(4, dtype=torch.complex64), "values", torch.complex64, dev, numel=5)
+    with pytest.raises(ValueError, match="shape"):
+        _check_tensor(torch.zeros((4, 4), dtype=torch.complex64), "out", torch.complex64, dev,
+                      shape=(4, 4, 4))
+    with pytest.raises(ValueError, match="is on"):
+        _check_tensor(torch.zeros(5, dtype=torch.complex64), "out", torch.complex64,
+                      torch.device("cuda:0"), numel=5)
