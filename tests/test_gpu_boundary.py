@@ -105,9 +105,14 @@ def _from_f16(a):
 
 
 def _check_f32(out, expected):
-    rel_re = np.abs(out.real.astype(np.float64) - expected.real) / np.maximum(np.abs(expected.real), 1e-30)
-    rel_im = np.abs(out.imag.astype(np.float64) - expected.imag) / np.maximum(np.abs(expected.imag), 1e-30)
-    assert max(rel_re.max(), rel_im.max()) < 1e-5
+    """Entry-wise relative error against the complex modulus, below 1e-5.
+    The reference (mod.rs:790-802) divides each component by its own size,
+    which holds there because its typed paths compute in FP64 and only the
+    storage is rounded; here every storage runs the FP32 device arithmetic,
+    whose error scales with |v|, so a component 300x smaller than |v| would
+    fail a per-component bound at an entry-wise error of 4e-8."""
+    err = np.abs(out.astype(np.complex128) - expected) / np.maximum(np.abs(expected), 1e-30)
+    assert err.max() < 1e-5
 
 
 def _check_f16(out16, expected):
