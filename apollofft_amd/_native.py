@@ -46,6 +46,7 @@ EXPORTS = [
     "anufft_last_error", "anufft_version",
     "anufft_debug_num_bins", "anufft_debug_sort", "anufft_debug_spread",
     "anufft_timing_enable", "anufft_timing_collect",
+    "anufft_execute_type1_noalloc", "anufft_scratch_capacity",
 ]
 
 SPREAD_AUTO, SPREAD_VALU, SPREAD_MFMA = 0, 1, 2
@@ -161,6 +162,8 @@ def lib():
         "anufft_workspace_size": (i32, [vp, i64, P(sz)]),
         "anufft_execute_type1": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
         "anufft_execute_type2": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
+        "anufft_execute_type1_noalloc": (i32, [vp, i64, P(vp), i64, vp, vp, vp]),
+        "anufft_scratch_capacity": (i32, [P(sz)]),
         "anufft_execute_grad_points": (i32, [vp, i64, P(vp), i64, vp, vp, P(vp), vp, sz, vp]),
         "anufft_execute_type2_backward": (i32, [vp, i64, P(vp), i64, vp, vp, vp, P(vp), vp, sz,
                                                 vp]),

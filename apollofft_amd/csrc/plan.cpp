@@ -848,6 +848,33 @@ anufft_status direct(int type, const anufft_plan *p, int64_t M, const float *con
   return ANUFFT_OK;
 }
 
+// Workspace of anufft_execute_type1_noalloc: one fixed array per GPU, held
+// by the loaded code object until the process ends.  Host code never passes
+// the symbol itself to a kernel (on the host it names a stand-in, not device
+// memory); it uses the per-device address from hipGetSymbolAddress, looked
+// up once per device with that device current.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // indexed by device id; nullptr until looked up
+
+// Device address of `scratch` on the current device `dev`.
+anufft_status scratch_address(int dev, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(ANUFFT_HIP_ERROR, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return ANUFFT_OK;
+}
+
 } // namespace
 
 // ============================================================================
@@ -1015,6 +1042,34 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.grid_to_modes((float2 *)out); });
   if (e != hipSuccess) return hip_fail(e, "fft");
   return ANUFFT_OK;
+}
+
+anufft_status anufft_scratch_capacity(size_t *bytes) {
+  if (!bytes) return fail(ANUFFT_INVALID_ARG, "NULL argument");
+  *bytes = kScratchBytes;
+  return ANUFFT_OK;
+}
+
+anufft_status anufft_execute_type1_noalloc(const anufft_plan *p, int64_t M,
+                                           const float *const coords[3], int64_t stride,
+                                           const float *values, float *out, void *stream) {
+  if (!p) return fail(ANUFFT_INVALID_ARG, "plan is NULL");
+  anufft_status st = check_points(M, stride);
+  if (st != ANUFFT_OK) return st;
+  const size_t need = layout(p, M).total;
+  if (need > kScratchBytes)
+    return fail(ANUFFT_CAPACITY_EXCEEDED,
+                "anufft_execute_type1_noalloc: %lld points on this plan need %zu workspace bytes, "
+                "more than its fixed %zu-byte scratch; use anufft_execute_type1 with a workspace",
+                (long long)M, need, kScratchBytes);
+  if (p->device < 0) return fail(ANUFFT_DEVICE_UNAVAILABLE, "no HIP device available for this plan");
+  void *ws = nullptr;
+  {
+    DeviceGuard guard(p->device);
+    st = scratch_address(p->device, &ws);
+  }
+  if (st != ANUFFT_OK) return st;
+  return anufft_execute_type1(p, M, coords, stride, values, out, ws, kScratchBytes, stream);
 }
 
 anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float *const coords[3],

@@ -139,6 +139,24 @@ anufft_status anufft_execute_type1(const anufft_plan *plan, int64_t num_points,
                                    const float *values, float *out, void *workspace,
                                    size_t workspace_bytes, void *hip_stream);
 
+/* Type-1 without a workspace argument and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and stream
+ * semantics as anufft_execute_type1; the workspace is a fixed scratch array
+ * of anufft_scratch_capacity() bytes (64 MiB) in device memory, one per GPU,
+ * held until the process ends.  Calls needing more than that
+ * (anufft_workspace_size(plan, num_points) > capacity; e.g. about 500k points
+ * on a 128^3 grid, about 800k on a 64^3 grid) return
+ * ANUFFT_CAPACITY_EXCEEDED.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap in time on one GPU (other streams, other threads) race
+ * and must be serialized by the caller. */
+anufft_status anufft_execute_type1_noalloc(const anufft_plan *plan, int64_t num_points,
+                                           const float *const coords[3], int64_t coord_stride,
+                                           const float *values, float *out, void *hip_stream);
+/* Size in bytes of the scratch array behind anufft_execute_type1_noalloc. */
+anufft_status anufft_scratch_capacity(size_t *bytes);
+
 /* Type-2, device pointers.  Replaces NufftPlan{1D,3D}::type2_into
  * (dimension_1d.rs:224, dimension_3d.rs:496) and execute_fast_type2_{1d,3d}
  * (device.rs:277,446).  modes: n or nx*ny*nz complex; out: M complex. */

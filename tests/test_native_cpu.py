@@ -235,3 +235,27 @@ def test_python_wrappers_reject_bad_tensors():
     with pytest.raises(ValueError, match="is on"):
         _check_tensor(torch.zeros(5, dtype=torch.complex64), "out", torch.complex64,
                       torch.device("cuda:0"), numel=5)
+
+
+def test_noalloc_entry_point_errors():
+    """anufft_execute_type1_noalloc rejects a NULL plan, bad point counts and
+    strides, and calls that need more than its fixed scratch (capacity
+    error naming the scratch), all before any device work (no GPU needed)."""
+    import ctypes
+    lib = N.lib()
+    cap = ctypes.c_size_t()
+    assert lib.anufft_scratch_capacity(ctypes.byref(cap)) == N.OK
+    assert cap.value == 64 << 20
+    assert lib.anufft_scratch_capacity(None) == N.INVALID_ARG
+    p = A.NufftPlan3D(64, 64, 64, 0.1, 0.1, 0.1, 2, 7)
+    buf = np.zeros(64, np.float32)
+    b = buf.ctypes.data
+    ptrs = N.ptr_array([b] * 3)
+    assert lib.anufft_execute_type1_noalloc(None, 3, ptrs, 1, b, b, None) == N.INVALID_ARG
+    assert lib.anufft_execute_type1_noalloc(p.handle, 2 ** 31, ptrs, 1, b, b, None) == N.U32_OVERFLOW
+    assert lib.anufft_execute_type1_noalloc(p.handle, 3, ptrs, 0, b, b, None) == N.INVALID_ARG
+    need = ctypes.c_size_t()
+    assert lib.anufft_workspace_size(p.handle, 600_000, ctypes.byref(need)) == N.OK
+    assert need.value > cap.value
+    assert lib.anufft_execute_type1_noalloc(p.handle, 600_000, ptrs, 1, b, b, None) == N.CAPACITY_EXCEEDED
+    assert "scratch" in lib.anufft_last_error().decode()
