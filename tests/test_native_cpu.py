@@ -32,6 +32,17 @@ def test_library_exports_every_header_symbol():
     assert lib.anufft_version().decode().startswith("apollo-nufft-hip")
 
 
+def test_rust_binding_covers_the_header():
+    """The Rust extern block of the integration guide declares exactly the
+    header's entry points."""
+    with open(os.path.join(ROOT, "INTEGRATION.md")) as fh:
+        text = fh.read()
+    block = text[text.index("extern \"C\" {"):]
+    block = block[:block.index("\n}\n")]
+    rust = sorted(set(re.findall(r"pub fn (anufft_[a-z0-9_]+)\s*\(", block)))
+    assert rust == _header_symbols()
+
+
 def _d(n):
     return 2 * np.pi / n
 
