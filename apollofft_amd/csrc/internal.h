@@ -25,6 +25,14 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;       // points per thread per chunk
 constexpr int kSortChunk = kSortThreads * kSortItems;
+// One-pass binning (stable counting sort on the whole tile key) when the
+// tile count fits the per-wave LDS count rows: kOneWaves waves per chunk,
+// each ranking kOneSteps x 64 consecutive points.
+constexpr int kOneWaves = 8;
+constexpr int kOneThreads = 64 * kOneWaves;
+constexpr int kOneSteps = 8;
+constexpr int kOneChunk = kOneThreads * kOneSteps;
+constexpr int kOneMaxBins = 9216;   // kOneWaves rows of 16-bit counts: 144 KiB of LDS
 constexpr int kSpreadBinX = 4;      // spreading tile x-extent (a lane holds bin_x + 2w - 1 cells)
 constexpr int kSpreadCols = 32;     // spreading tile (y,z) columns per axis, bin + 2w <= 32
 
@@ -48,7 +56,16 @@ struct Geom {
   int nbin[3];       // tiles per axis
   int pad[3];        // padded tile extents bin + 2w
   double scale[3];   // m / L: grid units per unit length
+  double inv_m[3];   // 1 / m (rounded once)
+  float inv_bin[3];  // 1 / bin (FP32): cell / bin = floor((cell + 0.5) inv_bin) for cells < 2^13
 };
+
+// cell / bin for 0 <= cell < 2^13, 1 <= bin < 2^13 without an integer
+// division: (cell + 1/2) / bin lies at least 1/(2 bin) from the integers
+// around it, far more than the FP32 rounding of the product (< 2^-10 / bin).
+__host__ __device__ __forceinline__ int cell_div(int cell, const Geom &g, int a) {
+  return (int)(((float)cell + 0.5f) * g.inv_bin[a]);
+}
 
 // Work of a subproblem for splitting the list among workgroups: one unit
 // per kWorkUnit points plus kSubWork units of fixed cost (staging, scale,
@@ -59,8 +76,8 @@ constexpr uint32_t kSubWork = 48;
 struct SortBuffers {
   uint32_t *keys[2];
   uint32_t *idx[2];
-  uint32_t *cnt;       // kRadix * nchunks digit counts, then offsets within each digit
-  uint32_t *scan_tmp;  // kRadix digit totals
+  uint32_t *cnt;       // per-chunk digit (one-pass: tile) counts, then offsets
+  uint32_t *scan_tmp;  // digit (one-pass: tile) totals
   uint32_t *bin_start; // nbins + 1
   uint32_t *sub_ofs;   // nbins + 2; [nbins] = number of subproblems, [nbins + 1] = total work
   int4 *subs;          // (bin, start, count, work before it)
@@ -154,6 +171,10 @@ hipError_t launch_fill(float2 *p, int64_t n, hipStream_t s);
 
 hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
                        hipStream_t s, const RecordOut *rec = nullptr);
+// Whether a tile count takes the one-pass binning, and the entries of
+// SortBuffers::cnt the binning of num_points points needs.
+bool sort_one_pass(int nbins);
+size_t sort_count_entries(int64_t num_points, int nbins);
 
 hipError_t launch_fft_pass(const FftPass &p, hipStream_t s);
 
@@ -217,16 +238,20 @@ size_t interp_lds_bytes(int w, const Geom &g, bool grad);
 // ---- device helpers ----------------------------------------------------------
 #if defined(__HIPCC__)
 // Fold a coordinate into [0, m) grid units and return the cell index.
-// t = x * scale, t -= floor(t/m) m, t += m if negative; g = min(floor(t), m-1).
-// q*m is exact (integer times integer below 2^53), so the subtraction rounds
-// once whether or not it is contracted into an FMA; the CPU replay of the
-// binning repeats this sequence in IEEE double and gets identical cells.
-__device__ __forceinline__ double fold_coord(float xf, double scale, int m, int &cell) {
+// t = x * scale, q = floor(t * (1/m)), t -= q m, then t += m if negative or
+// t -= m if t >= m (q may be off by one near a multiple of m: no FP64
+// division); cell = min(floor(t), m-1).  q*m is exact (integer times integer
+// below 2^53), so the subtraction rounds once whether or not it is
+// contracted into an FMA; the CPU replay of the binning repeats this
+// sequence in IEEE double and gets identical cells.
+__device__ __forceinline__ double fold_coord(float xf, double scale, int m, double inv_m,
+                                             int &cell) {
   double t = (double)xf * scale;
   double md = (double)m;
-  double q = floor(t / md);
+  double q = floor(t * inv_m);
   t = t - q * md;
   if (t < 0.0) t = t + md;
+  else if (t >= md) t = t - md;
   int c = (int)floor(t);
   if (c > m - 1) c = m - 1;
   if (c < 0) c = 0;
@@ -244,7 +269,7 @@ __device__ __forceinline__ void point_zrec(const PointArgs &pa, const Geom &g, i
   for (int a = 0; a < 3; ++a) {
     const float *cp = a == 0 ? pa.x : (a == 1 ? pa.y : pa.z);
     int cell;
-    const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], cell);
+    const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], g.inv_m[a], cell);
     z[a] = (float)(1.0 - 2.0 * (t - (double)cell));
     const int o = cell - (cell / g.bin[a]) * g.bin[a];
     off |= (a == 0 ? o : o + 1) << (8 * a);

@@ -265,10 +265,12 @@ Layout layout(const anufft_plan *p, int64_t M) {
     l.keys1 = take(4 * Mp);
     l.idx0 = take(4 * Mp);
     l.idx1 = take(4 * Mp);
-    int64_t cnt_n = (int64_t)kRadix * nch;
-    l.cnt = take(4 * (size_t)cnt_n);
     const int nb = std::max(p->bs.nbins, std::max(p->bi.nbins, p->bg.nbins));
-    l.scan = take(4 * (size_t)kRadix);  // digit totals of a pass
+    size_t cnt_n = 0;
+    for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
+      cnt_n = std::max(cnt_n, sort_count_entries(M, b->nbins));
+    l.cnt = take(4 * cnt_n);
+    l.scan = take(4 * (size_t)std::max(kRadix, nb));  // digit (or tile) totals
     l.bin_start = take(4 * (size_t)(nb + 1));
     l.sub_ofs = take(4 * (size_t)(nb + 2));
     int64_t nsub = 1;
@@ -473,9 +475,11 @@ void set_geom(anufft_plan::Binning &b, const anufft_plan *p, const int bin[3]) {
   for (int a = 0; a < 3; ++a) {
     g.m[a] = (int)p->m[a];
     g.scale[a] = (double)p->m[a] / p->L[a];
+    g.inv_m[a] = 1.0 / (double)p->m[a];
     g.bin[a] = std::max(1, std::min(bin[a], g.m[a]));
     g.nbin[a] = (g.m[a] + g.bin[a] - 1) / g.bin[a];
     g.pad[a] = g.bin[a] + 2 * p->w;
+    g.inv_bin[a] = 1.0f / (float)g.bin[a];
   }
   b.nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
   int bits = 1;

@@ -15,6 +15,8 @@
 //   subproblems (one block): scan of per-tile subproblem counts, list
 //   subs   : subproblems of at most max_sub points per tile, tiles in pencil
 //            order (x fastest)
+#include <algorithm>
+
 #include "internal.h"
 
 namespace anufft {
@@ -27,10 +29,10 @@ constexpr int kScanTile = kScanThreads * kScanItems;
 __device__ __forceinline__ uint32_t tile_key(const PointArgs &pa, const Geom &g, int dim,
                                              int64_t j) {
   int cx, cy = 0, cz = 0;
-  fold_coord(pa.x[j * pa.stride], g.scale[0], g.m[0], cx);
+  fold_coord(pa.x[j * pa.stride], g.scale[0], g.m[0], g.inv_m[0], cx);
   if (dim == 3) {
-    fold_coord(pa.y[j * pa.stride], g.scale[1], g.m[1], cy);
-    fold_coord(pa.z[j * pa.stride], g.scale[2], g.m[2], cz);
+    fold_coord(pa.y[j * pa.stride], g.scale[1], g.m[1], g.inv_m[1], cy);
+    fold_coord(pa.z[j * pa.stride], g.scale[2], g.m[2], g.inv_m[2], cz);
   }
   uint32_t bx = (uint32_t)(cx / g.bin[0]);
   uint32_t by = (uint32_t)(cy / g.bin[1]);
@@ -266,13 +268,29 @@ __device__ __forceinline__ int pencil_bin(int q, int3 nb) {
 // total work.  The entries of a round are written by the whole block, each
 // finding its tile by a binary search of the tiles' first entries (a tile of
 // a dense cluster can hold hundreds of subproblems).
+// With tot (the tile totals of the one-pass binning) the kernel first writes
+// bin_start[b] = sum of tot[d] over d < b, b in [0, nbins]; tot == NULL with
+// no points means all tiles are empty.
 constexpr int kSubThreads = 1024;
-__global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t *bin_start,
+__global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_start,
+                                                                  const uint32_t *tot, int from_tot,
                                                                   int nbins, int3 nb, int max_sub,
                                                                   uint32_t *nsub, int4 *subs) {
   __shared__ uint32_t sh[kSubThreads / 64];
   __shared__ uint32_t first[kSubThreads];  // first entry of each tile of the round
   __shared__ int4 tinfo[kSubThreads];      // tile: bin, first point, count, work before it
+  if (from_tot) {
+    uint32_t run = 0;
+    for (int base = 0; base <= nbins; base += kSubThreads) {
+      const int b = base + threadIdx.x;
+      const uint32_t v = (tot && b < nbins) ? tot[b] : 0u;
+      uint32_t total;
+      const uint32_t ex = block_scan_waves<kSubThreads>(v, sh, total);
+      if (b <= nbins) bin_start[b] = run + ex;
+      run += total;
+    }
+    __syncthreads();  // bin_start is read below by other threads of the block
+  }
   const uint32_t ms = (uint32_t)max_sub;
   const uint32_t wfull = (ms + kWorkUnit - 1) / kWorkUnit + kSubWork;  // a full subproblem
   uint32_t carry = 0, carry_w = 0;
@@ -316,10 +334,266 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(const uint32_t
   }
 }
 
+// ---- one-pass binning (tile count <= kOneMaxBins) ---------------------------
+// A stable counting sort on the whole tile key in one scatter: chunks of
+// kOneChunk points in input order, each split into kOneWaves contiguous runs
+// of kOneSteps x 64 points, one run per wave.
+//   chunk_count   : per-chunk tile histogram, cnt[chunk][tile]
+//   chunk_scan    : per-tile exclusive scan over the chunks (in place) and
+//                   the tile totals
+//   subproblems   : bin_start = exclusive scan of the totals, and the list
+//   chunk_scatter : each wave ranks its run's points among equal tiles with
+//                   ballots (earlier lanes, then earlier steps through a
+//                   per-wave LDS row of counts); the rows are scanned across
+//                   the waves; a point's place is bin_start[tile] + its
+//                   chunk's offset + its wave's offset + its rank.
+// Ties keep input order, so the result equals the LSD passes' (and
+// numpy.argsort(keys, kind="stable")).
+
+// Peers of this lane (lanes of the wave with the same key among `valid`
+// lanes), from one ballot per key bit.
+__device__ __forceinline__ uint64_t key_peers(uint32_t key, bool valid, int bits) {
+  uint64_t peers = __ballot(valid);
+  for (int b = 0; b < bits; ++b) {
+    const bool bit = (key >> b) & 1u;
+    const uint64_t m = __ballot(valid && bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// Coordinates of point j (zeros past the end); dim 1 reads x only.
+__device__ __forceinline__ void load_point(const PointArgs &pa, int dim, int64_t j, bool valid,
+                                           float (&c)[3]) {
+  c[0] = valid ? pa.x[j * pa.stride] : 0.f;
+  c[1] = (valid && dim == 3) ? pa.y[j * pa.stride] : 0.f;
+  c[2] = (valid && dim == 3) ? pa.z[j * pa.stride] : 0.f;
+}
+
+// Tile key of a point from its coordinates (tile_key's arithmetic); with
+// REC also the record fields of point_zrec (polynomial arguments, offsets).
+template <bool REC>
+__device__ __forceinline__ uint32_t fold_point(const float (&c)[3], const Geom &g, int dim,
+                                               float (&z)[3], int &off) {
+  int cell[3] = {0, 0, 0};
+  off = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (a > 0 && dim != 3) break;
+    const double t = fold_coord(c[a], g.scale[a], g.m[a], g.inv_m[a], cell[a]);
+    if (REC) z[a] = (float)(1.0 - 2.0 * (t - (double)cell[a]));
+  }
+  int b[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    b[a] = (a > 0 && dim != 3) ? 0 : cell_div(cell[a], g, a);
+    if (REC && (a == 0 || dim == 3)) {
+      const int o = cell[a] - b[a] * g.bin[a];
+      off |= (a == 0 ? o : o + 1) << (8 * a);
+    }
+  }
+  return ((uint32_t)b[0] * (uint32_t)g.nbin[1] + (uint32_t)b[1]) * (uint32_t)g.nbin[2] + (uint32_t)b[2];
+}
+
+// Tile histogram of one chunk: every thread loads its points' coordinates
+// first (all loads in flight at once), then folds them and counts each in
+// LDS (the sums do not depend on the order of the atomic adds).
+constexpr int kCountThreads = 1024;
+constexpr int kCountPer = kOneChunk / kCountThreads;
+__global__ __launch_bounds__(kCountThreads) void chunk_count_kernel(PointArgs pa, Geom g, int dim,
+                                                                    int nbins, uint32_t *cnt) {
+  extern __shared__ uint32_t hist[];  // nbins
+  for (int d = threadIdx.x; d < nbins; d += kCountThreads) hist[d] = 0;
+  const int64_t base = (int64_t)blockIdx.x * kOneChunk + threadIdx.x;
+  float c[kCountPer][3];
+#pragma unroll
+  for (int s = 0; s < kCountPer; ++s) {
+    const int64_t j = base + (int64_t)s * kCountThreads;
+    load_point(pa, dim, j, j < pa.num_points, c[s]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kCountPer; ++s) {
+    const int64_t j = base + (int64_t)s * kCountThreads;
+    const bool valid = j < pa.num_points;
+    float z[3];
+    int off;
+    if (valid) atomicAdd(&hist[fold_point<false>(c[s], g, dim, z, off)], 1u);
+  }
+  __syncthreads();
+  uint32_t *row = cnt + (int64_t)blockIdx.x * nbins;
+  for (int d = threadIdx.x; d < nbins; d += kCountThreads) row[d] = hist[d];
+}
+
+// cnt[c][d] -> sum of cnt[c'][d] over c' < c (in place); tot[d] = the total.
+// A block takes 64 tiles; its 16 waves take 16 segments of the chunks.
+constexpr int kChunkScanSegs = 16;
+__global__ __launch_bounds__(64 * kChunkScanSegs) void chunk_scan_kernel(uint32_t *cnt, int nchunks,
+                                                                         int nbins, uint32_t *tot) {
+  __shared__ uint32_t seg[kChunkScanSegs][64];
+  const int dl = threadIdx.x & 63, sg = threadIdx.x >> 6;
+  const int d = blockIdx.x * 64 + dl;
+  const int per = (nchunks + kChunkScanSegs - 1) / kChunkScanSegs;
+  const int c0 = min(nchunks, sg * per), c1 = min(nchunks, c0 + per);
+  uint32_t s = 0;
+  if (d < nbins) {
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c) s += cnt[(int64_t)c * nbins + d];
+  }
+  seg[sg][dl] = s;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kChunkScanSegs; ++k) {
+    const uint32_t v = seg[k][dl];
+    before += k < sg ? v : 0u;
+    total += v;
+  }
+  if (d < nbins) {
+#pragma unroll 8
+    for (int c = c0; c < c1; ++c) {
+      uint32_t *q = cnt + (int64_t)c * nbins + d;
+      const uint32_t v = *q;
+      *q = before;
+      before += v;
+    }
+    if (sg == 0) tot[d] = total;
+  }
+}
+
+template <bool REC>
+__global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(
+    PointArgs pa, Geom g, int dim, int nbins, int bits, const uint32_t *offs,
+    const uint32_t *bin_start, uint32_t *keys_out, uint32_t *idx_out, ZRec out,
+    const float2 *values) {
+  extern __shared__ uint16_t rows[];  // [kOneWaves][nbins]: per-wave tile counts, then offsets
+  for (int i = threadIdx.x; i < kOneWaves / 2 * nbins; i += kOneThreads) ((uint32_t *)rows)[i] = 0u;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint16_t *row = rows + wave * nbins;
+  const int64_t j0 = (int64_t)blockIdx.x * kOneChunk + (int64_t)wave * (64 * kOneSteps) + lane;
+  // the wave's coordinates, all loads in flight at once
+  float c[kOneSteps][3];
+#pragma unroll
+  for (int s = 0; s < kOneSteps; ++s) {
+    const int64_t j = j0 + 64 * s;
+    load_point(pa, dim, j, j < pa.num_points, c[s]);
+  }
+  __syncthreads();  // rows zeroed
+  uint32_t key[kOneSteps], rk[kOneSteps];
+  float zr[kOneSteps][3];
+  int off[kOneSteps];
+#pragma unroll
+  for (int s = 0; s < kOneSteps; ++s) {
+    const int64_t j = j0 + 64 * s;
+    const bool valid = j < pa.num_points;
+    const uint32_t k = valid ? fold_point<REC>(c[s], g, dim, zr[s], off[s]) : 0u;
+    const uint64_t peers = key_peers(k, valid, bits);
+    const uint64_t below = peers & lt;
+    uint32_t b0 = 0;
+    if (valid && below == 0ull) {  // the first lane of each key claims the step's slots
+      b0 = row[k];
+      row[k] = (uint16_t)(b0 + (uint32_t)__popcll(peers));
+    }
+    const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : lane;
+    b0 = __shfl(b0, leader);
+    key[s] = k;
+    rk[s] = b0 + (uint32_t)__popcll(below);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < nbins; d += kOneThreads) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kOneWaves; ++w) {
+      const uint32_t v = rows[w * nbins + d];
+      rows[w * nbins + d] = (uint16_t)run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  const uint32_t *coff = offs + (int64_t)blockIdx.x * nbins;
+  uint32_t pos[kOneSteps];
+  float2 val[kOneSteps];
+#pragma unroll
+  for (int s = 0; s < kOneSteps; ++s) {
+    const int64_t j = j0 + 64 * s;
+    const bool valid = j < pa.num_points;
+    const uint32_t k = key[s];
+    pos[s] = valid ? bin_start[k] + coff[k] + (uint32_t)row[k] + rk[s] : 0u;
+    val[s] = (REC && values && valid) ? values[j] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int s = 0; s < kOneSteps; ++s) {
+    const int64_t j = j0 + 64 * s;
+    if (j >= pa.num_points) continue;
+    if (REC) {
+      out.r[2 * (int64_t)pos[s]] = make_float4(zr[s][0], zr[s][1], zr[s][2], __int_as_float(off[s]));
+      out.r[2 * (int64_t)pos[s] + 1] =
+          make_float4(val[s].x, val[s].y, __uint_as_float(key[s]), __uint_as_float((uint32_t)j));
+    } else {
+      keys_out[pos[s]] = key[s];
+      idx_out[pos[s]] = (uint32_t)j;
+    }
+  }
+}
+
 } // namespace
+
+bool sort_one_pass(int nbins) { return nbins <= kOneMaxBins; }
+
+size_t sort_count_entries(int64_t num_points, int nbins) {
+  const int64_t M = std::max<int64_t>(num_points, 1);
+  const int64_t old_n = (int64_t)kRadix * ((M + kSortChunk - 1) / kSortChunk);
+  const int64_t one_n = sort_one_pass(nbins) ? (int64_t)nbins * ((M + kOneChunk - 1) / kOneChunk) : 0;
+  return (size_t)std::max(old_n, one_n);
+}
+
+// One-pass binning: see the comment above chunk_count_kernel.
+static hipError_t sort_points_one_pass(const PointArgs &pa, const Geom &g, int dim,
+                                       const SortBuffers &sb, hipStream_t s,
+                                       const RecordOut *rec) {
+  const int64_t n = pa.num_points;
+  const int nbins = sb.nbins;
+  int bits = 0;
+  while ((1 << bits) < nbins) ++bits;
+  const int nch = (int)((n + kOneChunk - 1) / kOneChunk);
+  if (n > 0) {
+    hipLaunchKernelGGL(chunk_count_kernel, dim3(nch), dim3(kCountThreads), sizeof(uint32_t) * nbins, s, pa,
+                       g, dim, nbins, sb.cnt);
+    hipLaunchKernelGGL(chunk_scan_kernel, dim3((nbins + 63) / 64), dim3(64 * kChunkScanSegs), 0,
+                       s, sb.cnt, nch, nbins, sb.scan_tmp);
+  }
+  // tile starts and the subproblem list need only the counts
+  const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
+  hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
+                     n > 0 ? (const uint32_t *)sb.scan_tmp : (const uint32_t *)nullptr, 1,
+                     sb.nbins, nb, sb.max_sub, sb.sub_ofs + sb.nbins, sb.subs);
+  if (n > 0) {
+    const size_t lds = sizeof(uint16_t) * (size_t)kOneWaves * nbins;
+    if (lds > 65536) {
+      const void *fn = rec ? (const void *)chunk_scatter_kernel<true>
+                           : (const void *)chunk_scatter_kernel<false>;
+      hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    ZRec out{};
+    if (rec) {
+      out = rec->out;
+      hipLaunchKernelGGL(chunk_scatter_kernel<true>, dim3(nch), dim3(kOneThreads), lds, s, pa, g,
+                         dim, nbins, bits, sb.cnt, sb.bin_start, (uint32_t *)nullptr,
+                         (uint32_t *)nullptr, out, rec->values);
+    } else {
+      hipLaunchKernelGGL(chunk_scatter_kernel<false>, dim3(nch), dim3(kOneThreads), lds, s, pa, g,
+                         dim, nbins, bits, sb.cnt, sb.bin_start, sb.keys[sb.passes & 1],
+                         sb.idx[sb.passes & 1], out, (const float2 *)nullptr);
+    }
+  }
+  return hipGetLastError();
+}
 
 hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
                        hipStream_t s, const RecordOut *rec) {
+  if (sort_one_pass(sb.nbins)) return sort_points_one_pass(pa, g, dim, sb, s, rec);
   const int64_t n = pa.num_points;
   // with records, the last pass writes rec->out; earlier ones alternate with tmp
   auto dst = [&](int q) { return ((sb.passes - 1 - q) & 1) == 0 ? rec->out : rec->tmp; };
@@ -360,7 +634,8 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
   hipLaunchKernelGGL(bounds_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sorted, n,
                      sb.bin_start, sb.nbins);
   hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
-                     sb.nbins, nb, sb.max_sub, sb.sub_ofs + sb.nbins, sb.subs);
+                     (const uint32_t *)nullptr, 0, sb.nbins, nb, sb.max_sub, sb.sub_ofs + sb.nbins,
+                     sb.subs);
   return hipGetLastError();
 }
 

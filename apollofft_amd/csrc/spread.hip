@@ -85,7 +85,7 @@ __device__ __forceinline__ void stage_points(const PointArgs &pa, const Geom &g,
     uint32_t j = pa.perm[start + p];
     const float *cp = ax == 0 ? pa.x : (ax == 1 ? pa.y : pa.z);
     int cell;
-    double t = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
+    double t = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], g.inv_m[ax], cell);
     float wt[T];
     int i0;
     if (DERIV) {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void point_prep_kernel(PointArgs pa, const flo
     for (int a = 0; a < 3; ++a) {
       const float *cp = a == 0 ? pa.x : (a == 1 ? pa.y : pa.z);
       int cell;
-      const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], cell);
+      const double t = fold_coord(cp[j * pa.stride], g.scale[a], g.m[a], g.inv_m[a], cell);
       kb_taps<W>(t, cell, poly, r + R::WX + a * 2 * W);
       const int o = cell - (cell / g.bin[a]) * g.bin[a];
       off |= (a == 0 ? o : o + 1) << (8 * a);
@@ -244,7 +244,7 @@ __device__ __forceinline__ void stage_spread(const PointArgs &pa, const Geom &g,
     const uint32_t j = pa.perm[start + p];
     const float *cp = ax == 1 ? pa.y : pa.z;
     int cell;
-    const double t = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], cell);
+    const double t = fold_coord(cp[(int64_t)j * pa.stride], g.scale[ax], g.m[ax], g.inv_m[ax], cell);
     const int l = cell - W + 1 - (ax == 1 ? oy : oz);  // first tap's column, in [1, bin]
     float *r = rows + p * kSpreadRow + (ax - 1) * kSpreadCols;
 #pragma unroll
@@ -1636,7 +1636,7 @@ __global__ void spread1d_kernel(const float *x, int64_t stride, int64_t n,
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x) {
     int cell;
-    double t = fold_coord(x[j * stride], g.scale[0], m, cell);
+    double t = fold_coord(x[j * stride], g.scale[0], m, g.inv_m[0], cell);
     float wt[T];
     int i0 = kb_taps<W>(t, cell, poly, wt);
     const float2 c = values[j];
@@ -1658,7 +1658,7 @@ __global__ void interp1d_kernel(const float *x, int64_t stride, int64_t n, const
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x) {
     int cell;
-    double t = fold_coord(x[j * stride], g.scale[0], m, cell);
+    double t = fold_coord(x[j * stride], g.scale[0], m, g.inv_m[0], cell);
     float wt[T];
     int i0 = kb_taps<W>(t, cell, poly, wt);
     float ar = 0.f, ai = 0.f;
@@ -1682,7 +1682,7 @@ __global__ void grad1d_kernel(const float *x, int64_t stride, int64_t n, const f
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
        j += (int64_t)gridDim.x * blockDim.x) {
     int cell;
-    double t = fold_coord(x[j * stride], g.scale[0], m, cell);
+    double t = fold_coord(x[j * stride], g.scale[0], m, g.inv_m[0], cell);
     float wt[T], dwt[T];
     int i0 = kb_taps_deriv<W>(t, cell, poly, wt, dwt);
     float dr = 0.f, di = 0.f;

@@ -1,7 +1,8 @@
 """CPU replay of the device point binning (numpy, IEEE double).
 
-The device computes, per axis, t = float64(x) * (m / L); t -= floor(t / m) m;
-t += m if t < 0; cell = min(floor(t), m - 1); tile = cell // bin; key =
+The device computes, per axis, t = float64(x) * (m / L); t -= floor(t * (1/m)) m;
+t += m if t < 0, else t -= m if t >= m; cell = min(floor(t), m - 1);
+tile = cell // bin; key =
 (tx * nby + ty) * nbz + tz, then a stable counting sort by key.  numpy never
 fuses a*b+c, and q*m is exact for integer q, so this replay reproduces the
 device keys and permutation bit for bit.
@@ -12,9 +13,9 @@ import numpy as np
 def fold_cells(xf32, n, d, m):
     scale = float(m) / (float(n) * float(d))
     t = np.asarray(xf32, dtype=np.float32).astype(np.float64) * scale
-    q = np.floor(t / float(m))
+    q = np.floor(t * (1.0 / float(m)))
     t = t - q * float(m)
-    t = np.where(t < 0.0, t + float(m), t)
+    t = np.where(t < 0.0, t + float(m), np.where(t >= float(m), t - float(m), t))
     c = np.floor(t).astype(np.int64)
     return np.clip(c, 0, m - 1)
 

@@ -270,3 +270,15 @@ def test_noalloc_entry_point_errors():
     assert need.value > cap.value
     assert lib.anufft_execute_type1_noalloc(p.handle, 600_000, ptrs, 1, b, b, None) == N.CAPACITY_EXCEEDED
     assert "scratch" in lib.anufft_last_error().decode()
+
+
+def test_fp32_cell_division_is_exact():
+    """The binning divides cells by tile sizes as floor((cell + 1/2) *
+    fp32(1/bin)) in FP32 (no integer division); for every cell and tile
+    size below 2^13 (grids are at most 4096 cells per axis) that equals the
+    integer quotient."""
+    cells = np.arange(8192, dtype=np.int64)
+    cf = cells.astype(np.float32) + np.float32(0.5)
+    for b in range(1, 8192):
+        q = (cf * (np.float32(1.0) / np.float32(b))).astype(np.int64)
+        assert np.array_equal(q, cells // b), b
