@@ -191,6 +191,9 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   float2 *tw = sh + 2 * (size_t)lb * Lf;  // twiddle table of length Lf
   int64_t *lbase = (int64_t *)(tw + Lf);  // [lb] input, [lb] output line offsets
   for (int i = threadIdx.x; i < Lf; i += kFftThreads) tw[i] = p.tw[i];
+  if (p.zero)
+    for (int i = blockIdx.x * kFftThreads + threadIdx.x; i < p.zero_n; i += gridDim.x * kFftThreads)
+      p.zero[i] = 0.f;
   const int64_t l0 = (int64_t)blockIdx.x * lb;
   const int nl = (int)((p.nlines - l0) < lb ? (p.nlines - l0) : lb);
   // element offsets of the block's lines, one 64-bit division per line
@@ -251,6 +254,15 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   __syncthreads();
 
   // ---- store (crop, conjugation, scale and deconvolution) ----
+  // with bin maxima: per (line, z bin) maxima in LDS (the free buffer; non-
+  // negative floats order like their bit patterns), then one global atomic
+  // maximum per (line, bin)
+  uint32_t *bm = (uint32_t *)dst;
+  const int nbz = p.bmax ? p.bm_nbin[2] : 0;
+  if (p.bmax) {
+    for (int i = threadIdx.x; i < nl * nbz; i += kFftThreads) bm[i] = 0u;
+    __syncthreads();
+  }
   const bool out_contig = p.out_si == 1;
   for (int idx = threadIdx.x; idx < nl * p.nout; idx += kFftThreads) {
     int li, o;
@@ -269,6 +281,16 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     v.x *= f;
     v.y *= f;
     p.out[lbase[lb + li] + (int64_t)o * p.out_si] = v;
+    if (p.bmax) atomicMax(&bm[li * nbz + o / p.bm_bin[2]], __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
+  }
+  if (p.bmax) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < nl * nbz; i += kFftThreads) {
+      const int li = i / nbz, zb = i - li * nbz;
+      const int64_t l = l0 + li, x = l / p.nb, y = l - x * p.nb;
+      const int64_t bin = ((x / p.bm_bin[0]) * p.bm_nbin[1] + y / p.bm_bin[1]) * p.bm_nbin[2] + zb;
+      atomicMax((uint32_t *)p.bmax + bin, bm[i]);
+    }
   }
 }
 

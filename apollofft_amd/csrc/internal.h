@@ -111,6 +111,15 @@ struct FftPass {
   int blue_n;
   const float2 *chirp;   // exp(-i pi n^2 / L), n in [0, L)
   const float2 *bhat;    // FFT_blue_n of the conjugate chirp filter, times 1/blue_n
+  // Bin maxima of the written grid (the last pass of a 3D Type-2, lines
+  // (a, b) = (x, y), output index = z): bmax[bin] = max |re|, |im| over the
+  // bin's cells, with bins of bm_bin cells (bm_nbin per axis); zero[0 ..
+  // zero_n) is cleared by the pass before (the first pass of the chain).
+  float *bmax;
+  int bm_bin[3];
+  int bm_nbin[3];
+  float *zero;
+  int zero_n;
 };
 
 constexpr int kDirectPrimeMax = 16;   // larger prime radices go through Bluestein
@@ -195,8 +204,10 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
 // MFMA interpolation needs 2 nbins floats of scratch (bin and tile maxima).
 // Tile maxima of the grid for the MFMA interpolation (no-op for the vector
 // ALU method); launch_interp3d reads them from the same scratch.
+// bin_max_done: the bin maxima (first half of the scratch) were written by the
+// last FFT pass (FftPass::bmax), so only the tile maxima are computed.
 hipError_t launch_tile_max3d(int w, int method, const float2 *grid, float *scratch, const Geom &g,
-                             hipStream_t s);
+                             hipStream_t s, bool bin_max_done = false);
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
                            const KbPoly *dpoly, int num_blocks, hipStream_t s);

@@ -177,8 +177,10 @@ struct StageTimer {
       pool.pop_back();
       return e;
     }
+    // timing only: no system-scope fence (cache writeback) when recorded,
+    // so timing a call does not slow it down
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
   }
 };
@@ -772,8 +774,10 @@ struct Exec {
     return launch_fft_pass(fx, s);
   }
 
-  // T2 modes -> grid: deconvolve, zero pad, inverse passes x, y, z.
-  hipError_t modes_to_grid(const float2 *modes, bool compat) const {
+  // T2 modes -> grid: deconvolve, zero pad, inverse passes x, y, z.  With
+  // bin_max (3D, MFMA interpolation) the passes also leave the bin maxima
+  // of the grid in the first half of the tile-maximum scratch.
+  hipError_t modes_to_grid(const float2 *modes, bool compat, bool bin_max = false) const {
     const int64_t mx = p->m[0], my = p->m[1], mz = p->m[2];
     const int64_t nx = p->n[0], ny = p->n[1], nz = p->n[2];
     float2 *g = grid(), *c = fftbuf();
@@ -786,6 +790,12 @@ struct Exec {
     // x: modes[nx][ny][nz] -> b[mx][ny][nz] (in the grid buffer)
     FftPass fx = make_pass(p, 0, modes, g, (int)nx, (int)mx, ny, nz, nz, 1, ny * nz, nz, 1, ny * nz,
                            true, 1.0);
+    const bool bm = bin_max && p->kernel_method == kSpreadMfma;
+    float *bmax = bm ? (float *)(ws + lay.tmax) : nullptr;
+    if (bm) {
+      fx.zero = bmax;
+      fx.zero_n = p->bi.nbins;
+    }
     if ((e = launch_fft_pass(fx, s)) != hipSuccess) return e;
     // y: b[mx][ny][nz] -> c[mx][my][nz]
     FftPass fy = make_pass(p, 1, g, c, (int)ny, (int)my, mx, nz, ny * nz, 1, nz, my * nz, 1, nz,
@@ -794,6 +804,13 @@ struct Exec {
     // z: c[mx][my][nz] -> grid[mx][my][mz]
     FftPass fz = make_pass(p, 2, c, g, (int)nz, (int)mz, mx, my, my * nz, nz, 1, my * mz, mz, 1,
                            true, scale);
+    if (bm) {
+      fz.bmax = bmax;
+      for (int a = 0; a < 3; ++a) {
+        fz.bm_bin[a] = p->bi.g.bin[a];
+        fz.bm_nbin[a] = p->bi.g.nbin[a];
+      }
+    }
     return launch_fft_pass(fz, s);
   }
 };
@@ -1094,12 +1111,12 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
         [&](const Exec &ea) {
           hipError_t r = ea.timed(ANUFFT_STAGE_FFT, [&] {
             return ea.modes_to_grid((const float2 *)modes,
-                                    p->opts.compat_reference_type2_3d_scaling != 0);
+                                    p->opts.compat_reference_type2_3d_scaling != 0, true);
           });
           if (r != hipSuccess) return r;
           return ea.timed(ANUFFT_STAGE_TILE_MAX, [&] {
             return launch_tile_max3d(p->w, p->kernel_method, ea.grid(),
-                                     (float *)(ea.ws + lay.tmax), p->bi.g, ea.s);
+                                     (float *)(ea.ws + lay.tmax), p->bi.g, ea.s, true);
           });
         },
         [&] { return ex.bin(p->bi, pa, nullptr, p->kernel_method == kSpreadMfma); });
@@ -1232,11 +1249,11 @@ anufft_status anufft_execute_type1_backward(const anufft_plan *p, int64_t M,
   PointArgs pa = ex.point_args(p->bi, M, coords, stride);
   hipError_t e = ex.fork_join(
       [&](const Exec &ea) {
-        hipError_t r = ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)G, false); });
+        hipError_t r = ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)G, false, true); });
         if (r != hipSuccess) return r;
         return ea.timed(ANUFFT_STAGE_TILE_MAX, [&] {
           return launch_tile_max3d(p->w, p->kernel_method, ea.grid(), (float *)(ea.ws + lay.tmax),
-                                   p->bi.g, ea.s);
+                                   p->bi.g, ea.s, true);
         });
       },
       [&] { return ex.bin(p->bi, pa, (const float2 *)values, true); });

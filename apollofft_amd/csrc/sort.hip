@@ -426,13 +426,18 @@ __global__ __launch_bounds__(kCountThreads) void chunk_count_kernel(PointArgs pa
 }
 
 // cnt[c][d] -> sum of cnt[c'][d] over c' < c (in place); tot[d] = the total.
-// A block takes 64 tiles; its 16 waves take 16 segments of the chunks.
-constexpr int kChunkScanSegs = 16;
-__global__ __launch_bounds__(64 * kChunkScanSegs) void chunk_scan_kernel(uint32_t *cnt, int nchunks,
-                                                                         int nbins, uint32_t *tot) {
-  __shared__ uint32_t seg[kChunkScanSegs][64];
-  const int dl = threadIdx.x & 63, sg = threadIdx.x >> 6;
-  const int d = blockIdx.x * 64 + dl;
+// A block takes 16 tiles; its 1024 threads split the chunks into 64
+// segments per tile (lane group of 16 = 16 consecutive tiles of one chunk),
+// so each thread has one short run of loads in flight.
+constexpr int kScanDigits = 16;
+constexpr int kChunkScanSegs = 64;
+__global__ __launch_bounds__(kScanDigits * kChunkScanSegs) void chunk_scan_kernel(uint32_t *cnt,
+                                                                                  int nchunks,
+                                                                                  int nbins,
+                                                                                  uint32_t *tot) {
+  __shared__ uint32_t seg[kChunkScanSegs][kScanDigits];
+  const int dl = threadIdx.x & (kScanDigits - 1), sg = threadIdx.x / kScanDigits;
+  const int d = blockIdx.x * kScanDigits + dl;
   const int per = (nchunks + kChunkScanSegs - 1) / kChunkScanSegs;
   const int c0 = min(nchunks, sg * per), c1 = min(nchunks, c0 + per);
   uint32_t s = 0;
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(64 * kChunkScanSegs) void chunk_scan_kernel(uint32_
   seg[sg][dl] = s;
   __syncthreads();
   uint32_t before = 0, total = 0;
-#pragma unroll
+#pragma unroll 16
   for (int k = 0; k < kChunkScanSegs; ++k) {
     const uint32_t v = seg[k][dl];
     before += k < sg ? v : 0u;
@@ -560,8 +565,8 @@ static hipError_t sort_points_one_pass(const PointArgs &pa, const Geom &g, int d
   if (n > 0) {
     hipLaunchKernelGGL(chunk_count_kernel, dim3(nch), dim3(kCountThreads), sizeof(uint32_t) * nbins, s, pa,
                        g, dim, nbins, sb.cnt);
-    hipLaunchKernelGGL(chunk_scan_kernel, dim3((nbins + 63) / 64), dim3(64 * kChunkScanSegs), 0,
-                       s, sb.cnt, nch, nbins, sb.scan_tmp);
+    hipLaunchKernelGGL(chunk_scan_kernel, dim3((nbins + kScanDigits - 1) / kScanDigits),
+                       dim3(kScanDigits * kChunkScanSegs), 0, s, sb.cnt, nch, nbins, sb.scan_tmp);
   }
   // tile starts and the subproblem list need only the counts
   const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);

@@ -940,55 +940,78 @@ __device__ __forceinline__ void load_tile(const float2 *grid, const Geom &g, int
 
 // ---- tile maxima of the grid (scales of the MFMA interpolation tiles) -------
 //
-// bmax[b] = largest |re| or |im| over the cells of bin b (one block per bin);
-// tmax[b] = largest bmax over the bins that the padded tile of bin b
-// overlaps, i.e. the largest component of that tile.
+// bmax[b] = largest |re| or |im| over the cells of bin b; tmax[b] = largest
+// bmax over the bins that the padded tile of bin b overlaps, i.e. an upper
+// bound of the largest component of that tile.
+//
+// Bin maxima: one block per (x bin, y bin) column of bins; its rows (x, y)
+// are read along z with a wave on 64 consecutive cells, each thread keeps
+// the maximum of one z cell over its rows, then folds it into the z bin's
+// slot in LDS (non-negative floats order like their bit patterns).
 __global__ __launch_bounds__(256) void bin_max_kernel(const float2 *grid, Geom g, float *bmax) {
-  __shared__ float red[4];
-  const int bin = blockIdx.x;
-  const int bz = bin % g.nbin[2];
-  const int by = (bin / g.nbin[2]) % g.nbin[1];
-  const int bx = bin / (g.nbin[2] * g.nbin[1]);
-  const int x0 = bx * g.bin[0], y0 = by * g.bin[1], z0 = bz * g.bin[2];
-  const int nx = min(g.bin[0], g.m[0] - x0), ny = min(g.bin[1], g.m[1] - y0),
-            nz = min(g.bin[2], g.m[2] - z0);
-  float mx = 0.f;
-  for (int i = threadIdx.x; i < nx * ny * nz; i += blockDim.x) {
-    const int z = i % nz, y = (i / nz) % ny, x = i / (nz * ny);
-    const float2 v = grid[((int64_t)(x0 + x) * g.m[1] + y0 + y) * g.m[2] + z0 + z];
-    mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-  }
-  mx = wave_max(mx);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  extern __shared__ uint32_t bm[];  // nbin[2]
+  const int bx = blockIdx.x / g.nbin[1], by = blockIdx.x - bx * g.nbin[1];
+  for (int i = threadIdx.x; i < g.nbin[2]; i += 256) bm[i] = 0u;
   __syncthreads();
-  if (threadIdx.x == 0) bmax[bin] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  const int x0 = bx * g.bin[0], y0 = by * g.bin[1];
+  const int nx = min(g.bin[0], g.m[0] - x0), ny = min(g.bin[1], g.m[1] - y0);
+  const int nrows = nx * ny, m2 = g.m[2];
+  const int tz = threadIdx.x & 63, tr = threadIdx.x >> 6;
+  for (int z = tz; z < m2; z += 64) {
+    float mx = 0.f;
+    for (int r = tr; r < nrows; r += 4) {
+      const int x = x0 + r / ny, y = y0 + r % ny;
+      const float2 v = grid[((int64_t)x * g.m[1] + y) * m2 + z];
+      mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    atomicMax(&bm[z / g.bin[2]], __float_as_uint(mx));
+  }
+  __syncthreads();
+  float *dst = bmax + ((int64_t)bx * g.nbin[1] + by) * g.nbin[2];
+  for (int i = threadIdx.x; i < g.nbin[2]; i += 256) dst[i] = __uint_as_float(bm[i]);
 }
 
-// One axis of the tile maxima, separably: dst[b] = max of src over the bins
-// (along `axis`) that the tile of bin b covers along that axis (cells
-// [b * bin + org, + ext), periodic).  Three launches (x, y, z) turn bin
-// maxima into tile maxima.
-__global__ __launch_bounds__(64) void tile_max_axis_kernel(const float *src, Geom g, int axis,
-                                                           int org, int ext, float *dst) {
-  const int bin = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
-  if (bin >= nbins) return;
-  int b3[3];
-  b3[2] = bin % g.nbin[2];
-  b3[1] = (bin / g.nbin[2]) % g.nbin[1];
-  b3[0] = bin / (g.nbin[2] * g.nbin[1]);
-  const int m = g.m[axis], bw = g.bin[axis];
-  int c = b3[axis] * bw + org;
+// Tile maxima: for the tile of bin b, the maximum of bmax over the bins that
+// meet its cells [b * bin + org, + ext) along each axis (periodic).  One
+// wave per tile: lanes 0-2 list the bins of one axis each in LDS, then the
+// lanes share the combinations.
+constexpr int kTileMaxList = 32;
+__device__ __forceinline__ int axis_bins(int b, int bw, int m, int org, int ext, int *out) {
+  int c = b * bw + org;
   while (c < 0) c += m;
   while (c >= m) c -= m;
-  float mx = 0.f;
-  for (int i = 0; i < ext; ++i) {
-    int n3[3] = {b3[0], b3[1], b3[2]};
-    n3[axis] = c / bw;
-    mx = fmaxf(mx, src[(n3[0] * g.nbin[1] + n3[1]) * g.nbin[2] + n3[2]]);
-    if (++c == m) c = 0;
+  int n = 0, left = ext;
+  while (left > 0 && n < kTileMaxList) {
+    const int bin = c / bw;
+    if (n == 0 || out[n - 1] != bin) out[n++] = bin;
+    const int step = min(left, min((bin + 1) * bw, m) - c);  // to the next bin or the grid end
+    left -= step;
+    c += step;
+    if (c >= m) c -= m;
   }
-  dst[bin] = mx;
+  return n;
+}
+
+__global__ __launch_bounds__(64) void tile_max_kernel(const float *bmax, Geom g, int3 org,
+                                                      int3 ext, float *tmax) {
+  __shared__ int lst[3][kTileMaxList];
+  __shared__ int cnt[3];
+  const int bin = blockIdx.x;
+  const int b3[3] = {bin / (g.nbin[2] * g.nbin[1]), (bin / g.nbin[2]) % g.nbin[1], bin % g.nbin[2]};
+  const int o3[3] = {org.x, org.y, org.z}, e3[3] = {ext.x, ext.y, ext.z};
+  if (threadIdx.x < 3) {
+    const int a = threadIdx.x;
+    cnt[a] = axis_bins(b3[a], g.bin[a], g.m[a], o3[a], e3[a], lst[a]);
+  }
+  __syncthreads();
+  const int nx = cnt[0], ny = cnt[1], nz = cnt[2];
+  float mx = 0.f;
+  for (int c = threadIdx.x; c < nx * ny * nz; c += 64) {
+    const int k = c % nz, j = (c / nz) % ny, i = c / (nz * ny);
+    mx = fmaxf(mx, bmax[((int64_t)lst[0][i] * g.nbin[1] + lst[1][j]) * g.nbin[2] + lst[2][k]]);
+  }
+  mx = wave_max(mx);
+  if (threadIdx.x == 0) tmax[bin] = mx;
 }
 
 // ---- MFMA interpolation -----------------------------------------------------
@@ -1813,19 +1836,22 @@ struct Spread3Batched {
 
 template <int W>
 struct TileMax3 {
-  static hipError_t run(const float2 *grid, float *bmax, const Geom &g, hipStream_t s) {
-    // bin maxima -> (x) -> (y) -> (z): tile maxima in the second half of
-    // the scratch, ping-ponging between its two halves
+  static hipError_t run(const float2 *grid, float *bmax, const Geom &g, hipStream_t s,
+                        bool bin_max_done) {
+    // bin maxima (first half of the scratch; unless the last FFT pass wrote
+    // them) -> tile maxima (second half)
     const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
     float *tmax = bmax + nbins;
     const unsigned tb = (unsigned)((nbins + 63) / 64);
-    hipLaunchKernelGGL(bin_max_kernel, dim3(nbins), dim3(256), 0, s, grid, g, bmax);
-    hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)bmax, g, 0,
-                       1 - W, MSpread<W>::PX, tmax);
-    hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)tmax, g, 1,
-                       -W, g.pad[1], bmax);
-    hipLaunchKernelGGL(tile_max_axis_kernel, dim3(tb), dim3(64), 0, s, (const float *)bmax, g, 2,
-                       -W, g.pad[2], tmax);
+    // tile_max_kernel lists at most kTileMaxList bins per axis
+    if (MSpread<W>::PX / g.bin[0] + 2 > kTileMaxList || g.pad[1] / g.bin[1] + 2 > kTileMaxList ||
+        g.pad[2] / g.bin[2] + 2 > kTileMaxList)
+      return hipErrorInvalidValue;
+    if (!bin_max_done)
+      hipLaunchKernelGGL(bin_max_kernel, dim3(g.nbin[0] * g.nbin[1]), dim3(256),
+                         sizeof(uint32_t) * g.nbin[2], s, grid, g, bmax);
+    hipLaunchKernelGGL(tile_max_kernel, dim3(nbins), dim3(64), 0, s, (const float *)bmax, g,
+                       make_int3(1 - W, -W, -W), make_int3(MSpread<W>::PX, g.pad[1], g.pad[2]), tmax);
     return hipGetLastError();
   }
 };
@@ -1990,9 +2016,9 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
 }
 
 hipError_t launch_tile_max3d(int w, int method, const float2 *grid, float *scratch, const Geom &g,
-                             hipStream_t s) {
+                             hipStream_t s, bool bin_max_done) {
   if (method != kSpreadMfma) return hipSuccess;
-  return dispatch_w<TileMax3>(w, grid, scratch, g, s);
+  return dispatch_w<TileMax3>(w, grid, scratch, g, s, bin_max_done);
 }
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float *scratch, float2 *out, const Geom &g, const KbPoly &poly,

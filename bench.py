@@ -19,11 +19,14 @@ on the device (anufft_direct_type1_select / anufft_direct_type2 /
 anufft_direct_type2_host, FP64) on sampled modes / points after timing.
 
 Printed: ONE JSON line with the metric, the pipeline roofline fraction and
-HBM fraction of the plain step, the roofline of its dominant kernel (average
-duration from HIP events recorded on the kernel's own stream during the
-timed steps) and the CPU baseline (the FP64 C restatement of apollo-nufft's
-algorithm, one thread and all host cores, on a bounded sample of the same
-workload).
+HBM fraction of the plain step, the roofline of its dominant kernel and the
+CPU baseline (the FP64 C restatement of apollo-nufft's algorithm, one thread
+and all host cores, on a bounded sample of the same workload).  The rate
+comes from K plain steps; the same K steps run once before them with HIP
+events recorded around every stage on the stream the stage runs on, which
+give the per-stage times and the dominant kernel's average duration (the
+event records themselves add a few microseconds between kernels, so they
+stay out of the rate).
 
 Other runs: --config sweep (100^3 -> 200^3 and 127^3 -> 254^3, M = 16e6
 clustered, --w 4/6/8), --config plain|grad|batched (one form only),
@@ -147,6 +150,10 @@ def bench_plain(args, torch, A, dev, steps, warmup):
 
     for _ in range(warmup):
         step()
+    # K steps with HIP events around every stage (on the stream each stage
+    # runs on): stage and kernel times; the event records add a few
+    # microseconds between kernels, so the rate comes from the K plain steps
+    # that follow
     torch.cuda.synchronize()
     plan.timing(True)
     torch.cuda.synchronize()
@@ -154,9 +161,15 @@ def bench_plain(args, torch, A, dev, steps, warmup):
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    el_instr = time.perf_counter() - t0
     stages = plan.timing_collect()
     plan.timing(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
 
     # per-type rates from separate timed loops of the same length
     per_type = {}
@@ -168,8 +181,8 @@ def bench_plain(args, torch, A, dev, steps, warmup):
             fn()
         torch.cuda.synchronize()
         per_type[name] = M * steps / (time.perf_counter() - ta) / 1e6
-    res = {"elapsed": el, "M": M, "n": n, "w": w, "meta": meta, "stages": stages,
-           "per_type": per_type}
+    res = {"elapsed": el, "elapsed_instrumented": el_instr, "M": M, "n": n, "w": w, "meta": meta,
+           "stages": stages, "per_type": per_type}
     if not args.no_check:
         res["err"] = {"type1": device_errors_t1(torch, plan, txyz, tc, out1, n, rng, 4096),
                       "type2": device_errors_t2(torch, plan, txyz, tf, out2, rng, 4096)}
@@ -526,6 +539,7 @@ def main():
         "metric": METRIC,
         "value": round(value, 2), "unit": "Mpts/s", "n_gpus": 1, "steps": K,
         "warmup": args.warmup, "ms_per_step": round(el / K * 1e3, 4),
+        "ms_per_step_stage_timed": round(r["elapsed_instrumented"] / K * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"3D Type-1 + Type-2 NUFFT, 64^3 modes -> 128^3 grid, KB w={r['w']}, "
