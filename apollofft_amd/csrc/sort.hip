@@ -260,33 +260,72 @@ __device__ __forceinline__ int pencil_bin(int q, int3 nb) {
   return (bx * nb.y + pz / nb.z) * nb.z + pz % nb.z;
 }
 
+// 64-bit exclusive scan over a block of NT threads (see block_scan_waves).
+template <int NT>
+__device__ __forceinline__ uint64_t block_scan_waves64(uint64_t v, uint64_t *sh, uint64_t &total) {
+  constexpr int NWV = NT / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[wave] = x;
+  __syncthreads();
+  uint64_t before = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < NWV; ++w) {
+    const uint64_t t = sh[w];
+    before += w < wave ? t : 0ull;
+    total += t;
+  }
+  __syncthreads();  // sh may be reused
+  return before + x - v;
+}
+
 // Subproblem list in one block: tiles in pencil order, ceil(count / max_sub)
-// subproblems each, exclusive scan of those counts in rounds of kSubThreads
-// tiles.  subs[i].w = work before subproblem i in list order, in units of
-// kWorkUnit points, a subproblem weighing ceil(count / kWorkUnit) +
-// kSubWork (its fixed cost); nsub[0] = number of subproblems, nsub[1] =
-// total work.  The entries of a round are written by the whole block, each
-// finding its tile by a binary search of the tiles' first entries (a tile of
-// a dense cluster can hold hundreds of subproblems).
-// With tot (the tile totals of the one-pass binning) the kernel first writes
-// bin_start[b] = sum of tot[d] over d < b, b in [0, nbins]; tot == NULL with
-// no points means all tiles are empty.
+// subproblems each, exclusive scan of those counts in rounds of kSubRound
+// tiles (kSubPer consecutive tiles per thread).  subs[i].w = work before
+// subproblem i in list order, in units of kWorkUnit points, a subproblem
+// weighing ceil(count / kWorkUnit) + kSubWork (its fixed cost); nsub[0] =
+// number of subproblems, nsub[1] = total work.  The subproblem counts and
+// the work are scanned together (one 64-bit scan: counts in the high word).
+// The entries of a round are written by the whole block, each finding its
+// tile by a binary search of the tiles' first entries (a tile of a dense
+// cluster can hold hundreds of subproblems).
+// With from_tot (the tile totals tot of the one-pass binning) the kernel
+// first writes bin_start[b] = sum of tot[d] over d < b, b in [0, nbins];
+// tot == NULL with no points means all tiles are empty.
 constexpr int kSubThreads = 1024;
+constexpr int kSubPer = 4;
+constexpr int kSubRound = kSubThreads * kSubPer;
 __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_start,
                                                                   const uint32_t *tot, int from_tot,
                                                                   int nbins, int3 nb, int max_sub,
                                                                   uint32_t *nsub, int4 *subs) {
   __shared__ uint32_t sh[kSubThreads / 64];
-  __shared__ uint32_t first[kSubThreads];  // first entry of each tile of the round
-  __shared__ int4 tinfo[kSubThreads];      // tile: bin, first point, count, work before it
+  __shared__ uint64_t sh64[kSubThreads / 64];
+  __shared__ uint32_t first[kSubRound];  // first entry of each tile of the round
+  __shared__ int4 tinfo[kSubRound];      // tile: bin, first point, count, work before it
   if (from_tot) {
     uint32_t run = 0;
-    for (int base = 0; base <= nbins; base += kSubThreads) {
-      const int b = base + threadIdx.x;
-      const uint32_t v = (tot && b < nbins) ? tot[b] : 0u;
+    for (int base = 0; base <= nbins; base += kSubRound) {
+      const int b0 = base + threadIdx.x * kSubPer;
+      uint32_t v[kSubPer], sum = 0;
+#pragma unroll
+      for (int i = 0; i < kSubPer; ++i) {
+        v[i] = (tot && b0 + i < nbins) ? tot[b0 + i] : 0u;
+        sum += v[i];
+      }
       uint32_t total;
-      const uint32_t ex = block_scan_waves<kSubThreads>(v, sh, total);
-      if (b <= nbins) bin_start[b] = run + ex;
+      uint32_t ex = run + block_scan_waves<kSubThreads>(sum, sh, total);
+#pragma unroll
+      for (int i = 0; i < kSubPer; ++i) {
+        if (b0 + i <= nbins) bin_start[b0 + i] = ex;
+        ex += v[i];
+      }
       run += total;
     }
     __syncthreads();  // bin_start is read below by other threads of the block
@@ -294,26 +333,39 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_
   const uint32_t ms = (uint32_t)max_sub;
   const uint32_t wfull = (ms + kWorkUnit - 1) / kWorkUnit + kSubWork;  // a full subproblem
   uint32_t carry = 0, carry_w = 0;
-  for (int base = 0; base < nbins; base += kSubThreads) {
-    const int q = base + threadIdx.x;
-    int b = 0;
-    uint32_t start = 0, c = 0, k = 0, wk = 0;
-    if (q < nbins) {
-      b = pencil_bin(q, nb);
-      start = bin_start[b];
-      c = bin_start[b + 1] - start;
-      k = (c + ms - 1) / ms;
-      wk = (c + kWorkUnit - 1) / kWorkUnit + k * kSubWork;
+  for (int base = 0; base < nbins; base += kSubRound) {
+    const int q0 = base + threadIdx.x * kSubPer;
+    int b[kSubPer];
+    uint32_t start[kSubPer], c[kSubPer], k[kSubPer], wk[kSubPer];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kSubPer; ++i) {
+      b[i] = 0;
+      start[i] = c[i] = k[i] = wk[i] = 0u;
+      if (q0 + i < nbins) {
+        b[i] = pencil_bin(q0 + i, nb);
+        start[i] = bin_start[b[i]];
+        c[i] = bin_start[b[i] + 1] - start[i];
+        k[i] = (c[i] + ms - 1) / ms;
+        wk[i] = (c[i] + kWorkUnit - 1) / kWorkUnit + k[i] * kSubWork;
+      }
+      sum += ((uint64_t)k[i] << 32) | wk[i];
     }
-    uint32_t total, total_w;
-    const uint32_t o = block_scan_waves<kSubThreads>(k, sh, total);
-    const uint32_t w = carry_w + block_scan_waves<kSubThreads>(wk, sh, total_w);
-    first[threadIdx.x] = o;
-    tinfo[threadIdx.x] = make_int4(b, (int)start, (int)c, (int)w);
+    uint64_t total64;
+    const uint64_t ex = block_scan_waves64<kSubThreads>(sum, sh64, total64);
+    uint32_t o = (uint32_t)(ex >> 32), w = carry_w + (uint32_t)ex;
+#pragma unroll
+    for (int i = 0; i < kSubPer; ++i) {
+      first[threadIdx.x * kSubPer + i] = o;
+      tinfo[threadIdx.x * kSubPer + i] = make_int4(b[i], (int)start[i], (int)c[i], (int)w);
+      o += k[i];
+      w += wk[i];
+    }
     __syncthreads();
+    const uint32_t total = (uint32_t)(total64 >> 32), total_w = (uint32_t)total64;
     for (uint32_t t = threadIdx.x; t < total; t += kSubThreads) {
       // owner = last tile with first <= t; it has t - first < its count
-      int lo = 0, hi = kSubThreads - 1;
+      int lo = 0, hi = kSubRound - 1;
       while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (first[mid] <= t) lo = mid;
