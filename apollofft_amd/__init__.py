@@ -12,12 +12,12 @@ from ._native import (CapacityExceeded, DeviceUnavailable, HipError, InvalidArgu
 from .api import (DEFAULT_NUFFT_KERNEL_WIDTH, DEFAULT_NUFFT_OVERSAMPLING, nufft_type1_1d,
                   nufft_type1_1d_fast, nufft_type1_3d, nufft_type1_3d_fast, nufft_type2_1d,
                   nufft_type2_1d_fast, nufft_type2_3d, nufft_type2_3d_fast)
-from .plan import NufftPlan1D, NufftPlan3D, execute_type1_batched
+from .plan import NufftPlan1D, NufftPlan3D, autotune3d, execute_type1_batched
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "NufftPlan1D", "NufftPlan3D", "execute_type1_batched",
+    "NufftPlan1D", "NufftPlan3D", "autotune3d", "execute_type1_batched",
     "nufft_type1_1d", "nufft_type2_1d", "nufft_type1_3d", "nufft_type2_3d",
     "nufft_type1_1d_fast", "nufft_type2_1d_fast", "nufft_type1_3d_fast", "nufft_type2_3d_fast",
     "DEFAULT_NUFFT_KERNEL_WIDTH", "DEFAULT_NUFFT_OVERSAMPLING",

@@ -46,7 +46,7 @@ EXPORTS = [
     "anufft_last_error", "anufft_version",
     "anufft_debug_num_bins", "anufft_debug_sort", "anufft_debug_spread",
     "anufft_timing_enable", "anufft_timing_collect",
-    "anufft_execute_type1_noalloc", "anufft_scratch_capacity",
+    "anufft_execute_type1_noalloc", "anufft_scratch_capacity", "anufft_autotune3d",
 ]
 
 SPREAD_AUTO, SPREAD_VALU, SPREAD_MFMA = 0, 1, 2
@@ -108,6 +108,9 @@ class Opts(ctypes.Structure):
         ("bin_size", ctypes.c_int32 * 3),
         ("max_subproblem", ctypes.c_int32),
         ("kernel_method", ctypes.c_int32),
+        ("sub_work", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("fft_lines", ctypes.c_int32),
     ]
 
 
@@ -130,6 +133,9 @@ class Meta(ctypes.Structure):
         ("poly_degree", ctypes.c_int32),
         ("poly_max_err", ctypes.c_double),
         ("kernel_method", ctypes.c_int32),
+        ("sub_work", ctypes.c_int32),
+        ("blocks_per_cu", ctypes.c_int32),
+        ("fft_lines", ctypes.c_int32),
     ]
 
 
@@ -164,6 +170,9 @@ def lib():
         "anufft_execute_type2": (i32, [vp, i64, P(vp), i64, vp, vp, vp, sz, vp]),
         "anufft_execute_type1_noalloc": (i32, [vp, i64, P(vp), i64, vp, vp, vp]),
         "anufft_scratch_capacity": (i32, [P(sz)]),
+        "anufft_autotune3d": (i32, [i64, i64, i64, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, i32, i32, P(Opts), i64, P(vp), i64, i32,
+                                    ctypes.c_char_p, P(Opts), P(ctypes.c_double)]),
         "anufft_execute_grad_points": (i32, [vp, i64, P(vp), i64, vp, vp, P(vp), vp, sz, vp]),
         "anufft_execute_type2_backward": (i32, [vp, i64, P(vp), i64, vp, vp, vp, P(vp), vp, sz,
                                                 vp]),

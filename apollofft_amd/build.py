@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libapollo_nufft_hip.so")
 ARCH = os.environ.get("APOLLO_NUFFT_ARCH", "gfx950")
 
-SOURCES = ["plan.cpp", "sort.hip", "fft.hip", "spread.hip", "direct.hip"]
+SOURCES = ["plan.cpp", "autotune.cpp", "sort.hip", "fft.hip", "spread.hip", "direct.hip"]
 
 # -packed-fp32-ops: the compiler must not form packed FP32 instructions.
 COMMON = [

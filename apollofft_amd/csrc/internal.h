@@ -85,6 +85,7 @@ struct SortBuffers {
   int nchunks;
   int passes;
   int max_sub;
+  int sub_work;        // fixed cost of a subproblem in kWorkUnit units
 };
 
 // One batched FFT pass over lines of a 3D (or 1D) array.

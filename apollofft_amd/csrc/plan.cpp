@@ -577,6 +577,17 @@ anufft_status finish_create(anufft_plan *p, anufft_plan **out) {
   return ANUFFT_OK;
 }
 
+// Launch settings of anufft_opts (speed only): 0 = automatic.
+anufft_status check_launch_opts(const anufft_opts *o) {
+  if (!o) return ANUFFT_OK;
+  if (o->sub_work < 0 || o->sub_work > 65536 || o->blocks_per_cu < 0 || o->blocks_per_cu > 16 ||
+      o->fft_lines < 0 || o->fft_lines > 4096)
+    return fail(ANUFFT_INVALID_ARG,
+                "launch settings out of range: sub_work %d (0..65536), blocks_per_cu %d (0..16), "
+                "fft_lines %d (0..4096)", o->sub_work, o->blocks_per_cu, o->fft_lines);
+  return ANUFFT_OK;
+}
+
 anufft_status check_common(int32_t sigma, int32_t w) {
   if (sigma < 2) return fail(ANUFFT_INVALID_ARG, "sigma must be >= 2 (got %d)", sigma);
   if (w < 2) return fail(ANUFFT_INVALID_ARG, "kernel_width must be >= 2 (got %d)", w);
@@ -629,6 +640,8 @@ FftPass make_pass(const anufft_plan *p, int axis, const float2 *in, float2 *out,
   const int Lf = f.blue_n ? f.blue_n : f.L;
   int lb = std::max(1, 1024 / Lf);
   while (lb > 8 && (f.nlines + lb - 1) / lb < 2 * (int64_t)p->cus) lb >>= 1;
+  // plan setting: lines per workgroup, within 64 KiB of LDS buffers
+  if (p->opts.fft_lines > 0) lb = std::max(1, std::min(p->opts.fft_lines, 4096 / Lf));
   f.lb = (int)std::max<int64_t>(1, std::min<int64_t>(lb, f.nlines));
   return f;
 }
@@ -728,6 +741,7 @@ struct Exec {
     sb.nchunks = lay.nchunks;
     sb.passes = b.passes;
     sb.max_sub = b.max_sub;
+    sb.sub_work = p->opts.sub_work > 0 ? p->opts.sub_work : (int)kSubWork;
     return sb;
   }
   PointArgs point_args(const anufft_plan::Binning &b, int64_t M, const float *const c[3],
@@ -744,7 +758,10 @@ struct Exec {
     pa.nsub = sb.sub_ofs + b.nbins;
     return pa;
   }
+  // Workgroups of a 3D point kernel: per_cu per CU (the plan's blocks_per_cu
+  // when set), at most one per subproblem.
   int blocks(const anufft_plan::Binning &b, int64_t M, int per_cu) const {
+    if (p->opts.blocks_per_cu > 0) per_cu = p->opts.blocks_per_cu;
     int64_t up = (int64_t)b.nbins + (M + b.max_sub - 1) / b.max_sub;
     int64_t n = std::min<int64_t>(up, (int64_t)per_cu * p->cus);
     return (int)std::max<int64_t>(n, 1);
@@ -814,6 +831,9 @@ struct Exec {
     return launch_fft_pass(fz, s);
   }
 };
+
+// Smallest subproblem (points) of the fused batched spreading.
+constexpr int kBatchMinSub = 4096;
 
 // Point counts: the subproblem list and the per-point records carry point
 // positions as signed 32-bit integers, so at most 2^31 - 1 points per call.
@@ -898,6 +918,11 @@ anufft_status scratch_address(int dev, void **out) {
 
 } // namespace
 
+namespace anufft {
+// Error report for the other translation units of the library (autotune.cpp).
+anufft_status report_error(anufft_status st, const char *msg) { return fail(st, "%s", msg); }
+} // namespace anufft
+
 // ============================================================================
 // C ABI
 // ============================================================================
@@ -924,6 +949,7 @@ anufft_status anufft_plan1d_create(int64_t n, double dx, int32_t sigma, int32_t 
   if (!std::isfinite(dx) || dx <= 0.0)
     return fail(ANUFFT_INVALID_ARG, "invalid NUFFT metadata field dx: %g (must be finite and > 0)", dx);
   anufft_status st = check_common(sigma, kernel_width);
+  if (st == ANUFFT_OK) st = check_launch_opts(opts);
   if (st != ANUFFT_OK) return st;
   if (n > (int64_t)UINT32_MAX / sigma) return fail(ANUFFT_U32_OVERFLOW, "oversampled length overflows u32");
   anufft_plan *p = new (std::nothrow) anufft_plan();
@@ -956,6 +982,7 @@ anufft_status anufft_plan3d_create(int64_t nx, int64_t ny, int64_t nz, double dx
     if (!std::isfinite(dd[a]) || dd[a] <= 0.0)
       return fail(ANUFFT_INVALID_ARG, "invalid NUFFT metadata field %s: %g (must be finite and > 0)", df[a], dd[a]);
   anufft_status st = check_common(sigma, kernel_width);
+  if (st == ANUFFT_OK) st = check_launch_opts(opts);
   if (st != ANUFFT_OK) return st;
   anufft_plan *p = new (std::nothrow) anufft_plan();
   if (!p) return fail(ANUFFT_INVALID_ARG, "out of host memory");
@@ -1012,6 +1039,9 @@ anufft_status anufft_plan_get_meta(const anufft_plan *p, anufft_meta *meta) {
   meta->poly_degree = kPolyCoefs - 1;
   meta->poly_max_err = p->poly_err;
   meta->kernel_method = p->kernel_method;
+  meta->sub_work = p->opts.sub_work > 0 ? p->opts.sub_work : (int)kSubWork;
+  meta->blocks_per_cu = p->opts.blocks_per_cu;
+  meta->fft_lines = p->opts.fft_lines;
   return ANUFFT_OK;
 }
 
@@ -1483,11 +1513,19 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   // zeroing and the odd problems on the first plan's second stream, even
   // problems on the caller's (small problems are launch-bound, so two
   // streams keep more of them in flight)
+  // one persistent launch splits the problems' combined work prefix, so
+  // every problem weighs its subproblems alike, with the automatic weight
+  // (the plans' own settings are tuned for single calls)
+  const int sub_work = (int)kSubWork;
   auto sort_range = [&](int parity, hipStream_t st) {
     for (int i = parity; i < nprob; i += 2) {
       const RecordOut ro{probs[i].zr, ex[i].zrec(num_points[i], true), (const float2 *)values[i]};
-      hipError_t r = sort_points(probs[i].pa, plans[i]->bs.g, 3, ex[i].sort_buffers(plans[i]->bs),
-                                 st, mf ? &ro : nullptr);
+      SortBuffers sb = ex[i].sort_buffers(plans[i]->bs);
+      sb.sub_work = sub_work;
+      // the persistent launch balances work across all problems itself:
+      // smaller subproblems only add fixed costs there
+      sb.max_sub = std::max(sb.max_sub, kBatchMinSub);
+      hipError_t r = sort_points(probs[i].pa, plans[i]->bs.g, 3, sb, st, mf ? &ro : nullptr);
       if (r != hipSuccess) return r;
     }
     return hipSuccess;

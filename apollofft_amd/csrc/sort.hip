@@ -304,7 +304,8 @@ constexpr int kSubRound = kSubThreads * kSubPer;
 __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_start,
                                                                   const uint32_t *tot, int from_tot,
                                                                   int nbins, int3 nb, int max_sub,
-                                                                  uint32_t *nsub, int4 *subs) {
+                                                                  int sub_work, uint32_t *nsub,
+                                                                  int4 *subs) {
   __shared__ uint32_t sh[kSubThreads / 64];
   __shared__ uint64_t sh64[kSubThreads / 64];
   __shared__ uint32_t first[kSubRound];  // first entry of each tile of the round
@@ -331,7 +332,8 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_
     __syncthreads();  // bin_start is read below by other threads of the block
   }
   const uint32_t ms = (uint32_t)max_sub;
-  const uint32_t wfull = (ms + kWorkUnit - 1) / kWorkUnit + kSubWork;  // a full subproblem
+  const uint32_t sw = (uint32_t)sub_work;
+  const uint32_t wfull = (ms + kWorkUnit - 1) / kWorkUnit + sw;  // a full subproblem
   uint32_t carry = 0, carry_w = 0;
   for (int base = 0; base < nbins; base += kSubRound) {
     const int q0 = base + threadIdx.x * kSubPer;
@@ -347,7 +349,7 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_
         start[i] = bin_start[b[i]];
         c[i] = bin_start[b[i] + 1] - start[i];
         k[i] = (c[i] + ms - 1) / ms;
-        wk[i] = (c[i] + kWorkUnit - 1) / kWorkUnit + k[i] * kSubWork;
+        wk[i] = (c[i] + kWorkUnit - 1) / kWorkUnit + k[i] * sw;
       }
       sum += ((uint64_t)k[i] << 32) | wk[i];
     }
@@ -624,7 +626,7 @@ static hipError_t sort_points_one_pass(const PointArgs &pa, const Geom &g, int d
   const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
   hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
                      n > 0 ? (const uint32_t *)sb.scan_tmp : (const uint32_t *)nullptr, 1,
-                     sb.nbins, nb, sb.max_sub, sb.sub_ofs + sb.nbins, sb.subs);
+                     sb.nbins, nb, sb.max_sub, sb.sub_work, sb.sub_ofs + sb.nbins, sb.subs);
   if (n > 0) {
     const size_t lds = sizeof(uint16_t) * (size_t)kOneWaves * nbins;
     if (lds > 65536) {
@@ -691,8 +693,8 @@ hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBu
   hipLaunchKernelGGL(bounds_kernel, dim3((nb1 + 255) / 256), dim3(256), 0, s, sorted, n,
                      sb.bin_start, sb.nbins);
   hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
-                     (const uint32_t *)nullptr, 0, sb.nbins, nb, sb.max_sub, sb.sub_ofs + sb.nbins,
-                     sb.subs);
+                     (const uint32_t *)nullptr, 0, sb.nbins, nb, sb.max_sub, sb.sub_work,
+                     sb.sub_ofs + sb.nbins, sb.subs);
   return hipGetLastError();
 }
 

@@ -33,6 +33,9 @@ def _meta_dict(m):
         "poly_degree": m.poly_degree,
         "poly_max_err": m.poly_max_err,
         "kernel_method": {1: "valu", 2: "mfma"}.get(m.kernel_method, m.kernel_method),
+        "sub_work": m.sub_work,
+        "blocks_per_cu": m.blocks_per_cu,
+        "fft_lines": m.fft_lines,
     }
 
 
@@ -62,6 +65,73 @@ def _check_tensor(t, name, dtype, device, shape=None, numel=None):
         raise ValueError(f"{name} must have {numel} elements (got {t.numel()})")
 
 
+_TUNED = ("kernel_method", "max_subproblem", "sub_work", "blocks_per_cu", "fft_lines")
+_METHODS = {None: N.SPREAD_AUTO, "auto": N.SPREAD_AUTO, "valu": N.SPREAD_VALU,
+            "mfma": N.SPREAD_MFMA}
+
+
+def _make_opts(grid_policy="reference", compat=False, device_id=None, bin_size=None,
+               max_subproblem=None, kernel_method=None, sub_work=None, blocks_per_cu=None,
+               fft_lines=None):
+    o = N.default_opts()
+    o.kernel_method = _METHODS[kernel_method]
+    o.grid_policy = _policy(grid_policy)
+    o.compat_reference_type2_3d_scaling = int(bool(compat))
+    o.device_id = -1 if device_id is None else int(device_id)
+    if bin_size is not None:
+        b = (bin_size,) * 3 if isinstance(bin_size, int) else tuple(bin_size)
+        for a in range(3):
+            o.bin_size[a] = int(b[a])
+    if max_subproblem:
+        o.max_subproblem = int(max_subproblem)
+    if sub_work:
+        o.sub_work = int(sub_work)
+    if blocks_per_cu:
+        o.blocks_per_cu = int(blocks_per_cu)
+    if fft_lines:
+        o.fft_lines = int(fft_lines)
+    return o
+
+
+def autotune3d(nx, ny, nz, dx, dy, dz, sigma=2, kernel_width=6, num_points=1_000_000,
+               positions=None, workload="both", cache=None, grid_policy="reference",
+               kernel_method=None, device_id=None):
+    """Launch settings of a 3D plan picked by timing (anufft_autotune3d).
+
+    positions: optional (M, 3) float32 device tensor of the points to tune
+    on (num_points is then M); otherwise num_points uniform synthetic
+    points.  workload: "type1", "type2" or "both".  cache: path of the JSON
+    cache (looked up first, written after tuning).  Returns a dict of the
+    settings (pass it as ``tuned=`` to NufftPlan3D) and "ms", the time of
+    one workload with them."""
+    o = _make_opts(grid_policy, False, device_id, None, None, kernel_method)
+    coords, stride = None, 0
+    if positions is not None:
+        if positions.dtype != _torch().float32 or positions.dim() != 2 or positions.shape[1] != 3:
+            raise ValueError("positions must be an (M, 3) float32 tensor")
+        if not positions.is_contiguous() or positions.device.type != "cuda":
+            raise ValueError("positions must be a contiguous device tensor")
+        num_points = positions.shape[0]
+        b = positions.data_ptr()
+        coords, stride = N.ptr_array([b, b + 4, b + 8]), 3
+    wl = {"type1": 1, "type2": 2, "both": 3}[workload]
+    out = N.Opts()
+    ms = ctypes.c_double()
+    N.check(N.lib().anufft_autotune3d(int(nx), int(ny), int(nz), float(dx), float(dy), float(dz),
+                                      int(sigma), int(kernel_width), ctypes.byref(o),
+                                      int(num_points), coords, int(stride), wl,
+                                      None if cache is None else str(cache).encode(),
+                                      ctypes.byref(out), ctypes.byref(ms)), "autotune3d")
+    res = {k: int(getattr(out, k)) for k in _TUNED}
+    res["ms"] = ms.value
+    return res
+
+
+def _torch():
+    import torch
+    return torch
+
+
 class _PlanBase:
     dim = 0
 
@@ -70,19 +140,13 @@ class _PlanBase:
         self._ws = None
         self._ws_cap = -1
 
-    def _opts(self, grid_policy, compat, device_id, bin_size, max_subproblem, kernel_method=None):
-        o = N.default_opts()
-        o.kernel_method = {None: N.SPREAD_AUTO, "auto": N.SPREAD_AUTO, "valu": N.SPREAD_VALU,
-                           "mfma": N.SPREAD_MFMA}[kernel_method]
-        o.grid_policy = _policy(grid_policy)
-        o.compat_reference_type2_3d_scaling = int(bool(compat))
-        o.device_id = -1 if device_id is None else int(device_id)
-        if bin_size is not None:
-            b = (bin_size,) * 3 if isinstance(bin_size, int) else tuple(bin_size)
-            for a in range(3):
-                o.bin_size[a] = int(b[a])
-        if max_subproblem:
-            o.max_subproblem = int(max_subproblem)
+    def _opts(self, grid_policy, compat, device_id, bin_size, max_subproblem, kernel_method=None,
+              sub_work=None, blocks_per_cu=None, fft_lines=None, tuned=None):
+        o = _make_opts(grid_policy, compat, device_id, bin_size, max_subproblem, kernel_method,
+                       sub_work, blocks_per_cu, fft_lines)
+        if tuned:
+            for k in _TUNED:
+                setattr(o, k, int(tuned[k]))
         return o
 
     @property
@@ -402,11 +466,12 @@ class NufftPlan3D(_PlanBase):
 
     def __init__(self, nx, ny, nz, dx, dy, dz, sigma=2, kernel_width=6,
                  grid_policy="reference", compat_reference_type2_3d_scaling=False,
-                 device_id=None, bin_size=None, max_subproblem=None, kernel_method=None):
+                 device_id=None, bin_size=None, max_subproblem=None, kernel_method=None,
+                 sub_work=None, blocks_per_cu=None, fft_lines=None, tuned=None):
         super().__init__()
         h = ctypes.c_void_p()
         o = self._opts(grid_policy, compat_reference_type2_3d_scaling, device_id, bin_size,
-                       max_subproblem, kernel_method)
+                       max_subproblem, kernel_method, sub_work, blocks_per_cu, fft_lines, tuned)
         N.check(N.lib().anufft_plan3d_create(int(nx), int(ny), int(nz), float(dx), float(dy),
                                              float(dz), int(sigma), int(kernel_width),
                                              ctypes.byref(o), ctypes.byref(h)), "plan3d_create")

@@ -103,6 +103,12 @@ def load_traffic(kernel):
         return None
 
 
+def tune_cache():
+    """A fresh JSON cache file for the autotuner (nothing carries over between runs)."""
+    import tempfile
+    return os.path.join(tempfile.mkdtemp(prefix="anufft_tune_"), "tune.json")
+
+
 def host_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -135,11 +141,15 @@ def bench_plain(args, torch, A, dev, steps, warmup):
     rng = np.random.default_rng(1234)
     xyz, c, f = make_inputs(rng, M, n)
     d = (TWO_PI / n[0],) * 3
-    plan = A.NufftPlan3D(*n, *d, 2, w, kernel_method=args.method)
-    meta = plan.meta
     txyz = torch.from_numpy(xyz).to(dev)
     tc = torch.from_numpy(c).to(dev)
     tf = torch.from_numpy(f).to(dev)
+    # launch settings from the autotuner, timed on these points (untimed here)
+    tuned = None if args.no_autotune else A.autotune3d(
+        *n, *d, 2, w, positions=txyz, workload="both", cache=tune_cache(),
+        kernel_method=None if args.method == "auto" else args.method)
+    plan = A.NufftPlan3D(*n, *d, 2, w, kernel_method=args.method, tuned=tuned)
+    meta = plan.meta
     out1 = torch.empty(n, dtype=torch.complex64, device=dev)
     out2 = torch.empty(M, dtype=torch.complex64, device=dev)
     plan.workspace(M, dev)
@@ -182,7 +192,7 @@ def bench_plain(args, torch, A, dev, steps, warmup):
         torch.cuda.synchronize()
         per_type[name] = M * steps / (time.perf_counter() - ta) / 1e6
     res = {"elapsed": el, "elapsed_instrumented": el_instr, "M": M, "n": n, "w": w, "meta": meta,
-           "stages": stages, "per_type": per_type}
+           "stages": stages, "per_type": per_type, "tuned": tuned}
     if not args.no_check:
         res["err"] = {"type1": device_errors_t1(torch, plan, txyz, tc, out1, n, rng, 4096),
                       "type2": device_errors_t2(torch, plan, txyz, tf, out2, rng, 4096)}
@@ -267,17 +277,27 @@ def bench_batched(args, torch, A, dev, steps, warmup):
     if args.batch_points:
         Ms = np.minimum(Ms, args.batch_points)
     plans, pos, vals, outs = [], [], [], []
-    plan_cache = {}
     for nn, M in zip(sizes, Ms):
         nn = int(nn)
-        if nn not in plan_cache:
-            plan_cache[nn] = A.NufftPlan3D(nn, nn, nn, *(TWO_PI / nn,) * 3, 2, w, grid_policy="exact")
-        plans.append(plan_cache[nn])
         xyz = radial_points(rng, int(M))
         c = (rng.standard_normal(int(M)) + 1j * rng.standard_normal(int(M))).astype(np.complex64)
         pos.append(torch.from_numpy(xyz).to(dev))
         vals.append(torch.from_numpy(c).to(dev))
         outs.append(torch.empty((nn, nn, nn), dtype=torch.complex64, device=dev))
+    # one plan per mode count; launch settings from the autotuner, timed on
+    # the largest problem of that size (untimed here)
+    plan_cache, tuned = {}, {}
+    cache = tune_cache()
+    for nn in sorted(set(int(v) for v in sizes)):
+        d = (TWO_PI / nn,) * 3
+        t = None
+        if not args.no_autotune:
+            big = max((i for i in range(nprob) if int(sizes[i]) == nn), key=lambda i: int(Ms[i]))
+            t = A.autotune3d(nn, nn, nn, *d, 2, w, positions=pos[big], workload="type1",
+                             cache=cache, grid_policy="exact")
+            tuned[str(nn)] = t
+        plan_cache[nn] = A.NufftPlan3D(nn, nn, nn, *d, 2, w, grid_policy="exact", tuned=t)
+    plans = [plan_cache[int(nn)] for nn in sizes]
     total = int(Ms.sum())
     seq_outs = [torch.empty_like(o) for o in outs]
 
@@ -323,13 +343,20 @@ def bench_batched(args, torch, A, dev, steps, warmup):
         batched()
     torch.cuda.synchronize()
     t_bat = (time.perf_counter() - t0) / steps
+    # stage times of one eager batched call (HIP events on its streams)
+    plans[0].timing(True)
+    A.execute_type1_batched(plans, pos, vals, outs)
+    torch.cuda.synchronize()
+    bst = plans[0].timing_collect()
+    plans[0].timing(False)
     row = {"workload": f"{nprob} Type-1 problems, modes 32^3/48^3/64^3/96^3 on exact grids, "
                        f"KB w={w}, {total:,} points on radial lines (~1/r^2)",
            "problems": nprob, "points": total, "kernel_width": w, "graph": graph is not None,
            "batched_ms": round(t_bat * 1e3, 3), "sequential_ms": round(t_seq * 1e3, 3),
            "batched_mpts_per_s": round(total / t_bat / 1e6, 1),
            "sequential_mpts_per_s": round(total / t_seq / 1e6, 1),
-           "speedup_vs_sequential": round(t_seq / t_bat, 4)}
+           "speedup_vs_sequential": round(t_seq / t_bat, 4), "tuned": tuned or None,
+           "batched_stage_ms": {k: round(v[0], 3) for k, v in bst.items() if v[0]}}
     diffs = [rel_l2(o.cpu().numpy(), so.cpu().numpy()) for o, so in zip(outs, seq_outs)]
     row["max_rel_diff_batched_vs_sequential"] = max(diffs)
     if not args.no_check:
@@ -462,6 +489,8 @@ def main():
     ap.add_argument("--batch-points", type=int, default=0,
                     help="cap the points of each batched problem")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="automatic launch settings instead of the autotuner's")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--method", default="auto", choices=["auto", "valu", "mfma"],
                     help="3D spreading / interpolation kernels")
@@ -545,7 +574,8 @@ def main():
         "config": {"workload": f"3D Type-1 + Type-2 NUFFT, 64^3 modes -> 128^3 grid, KB w={r['w']}, "
                                f"M={M:,} uniform points, complex64",
                    "modes": list(r["n"]), "grid": list(r["meta"]["m"]), "kernel_width": r["w"],
-                   "points": M, "bin_size": list(r["meta"]["bin_size"])},
+                   "points": M, "bin_size": list(r["meta"]["bin_size"]),
+                   "launch_settings": r["tuned"]},
         "mpts_per_s_by_type": {k: round(v, 2) for k, v in r["per_type"].items()},
         "stage_ms_per_step": {k: round(v, 4) for k, v in per_step.items()},
         "rel_err_vs_fp64_direct": r.get("err"),

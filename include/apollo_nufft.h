@@ -71,6 +71,13 @@ typedef struct anufft_opts {
   int32_t kernel_method;                     /* 3D spreading and interpolation kernels:
                                                 0 automatic, 1 vector ALU (register columns,
                                                 LDS tiles), 2 FP16-split MFMA blocks */
+  /* Launch settings that change speed, never results (0 = automatic; the
+   * values anufft_autotune3d picks): */
+  int32_t sub_work;      /* fixed cost of a subproblem, in units of 16 points, when the
+                            subproblem list is split among workgroups (automatic: 48) */
+  int32_t blocks_per_cu; /* workgroups per CU of the 3D point kernels (automatic: per kernel) */
+  int32_t fft_lines;     /* lines per workgroup of the FFT passes (automatic: 1024 / length,
+                            at least 2 workgroups per CU) */
 } anufft_opts;
 
 typedef struct anufft_meta {
@@ -91,6 +98,9 @@ typedef struct anufft_meta {
   int32_t poly_degree;  /* degree of the per-tap kernel polynomials */
   double poly_max_err;  /* max |phi - fit| on [-w, w] measured at plan creation */
   int32_t kernel_method; /* 3D spread/interp kernels in use (1 vector ALU, 2 MFMA) */
+  int32_t sub_work;      /* launch settings in use (see anufft_opts) */
+  int32_t blocks_per_cu; /* 0: automatic */
+  int32_t fft_lines;     /* 0: automatic */
 } anufft_meta;
 
 typedef struct anufft_plan anufft_plan;
@@ -229,6 +239,29 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
                                            const float *const *values, float *const *out,
                                            void *workspace, size_t workspace_bytes,
                                            void *hip_stream);
+
+/* Autotuner of the launch settings of a 3D plan geometry.  No counterpart
+ * in the reference, whose binning follows a fixed rule (bucket_count,
+ * apollo-nufft/src/infrastructure/kernel/kaiser_bessel.rs:162-164).  Times
+ * candidate values of kernel_method (when base->kernel_method is 0),
+ * max_subproblem, sub_work, blocks_per_cu and fft_lines by coordinate
+ * descent, each candidate a plan built with them and timed with HIP events
+ * (best of three runs) on `workload` (1 Type-1, 2 Type-2, 3 both) over
+ * num_points points: the caller's device coordinates (coords, stride as
+ * for execute) or, with coords NULL, uniform synthetic points.  *tuned =
+ * *base (defaults when NULL) with the fastest settings; *best_ms = its
+ * time per workload.  With cache_path the result is first looked up in and
+ * then stored to a JSON file keyed by device, grid, kernel width, sigma,
+ * method, point count (rounded up to a power of two), workload and point
+ * source.  Allocates device memory and synchronizes (not for hot loops);
+ * results of the tuned plan equal those of any other settings up to
+ * floating-point summation order. */
+anufft_status anufft_autotune3d(int64_t nx, int64_t ny, int64_t nz, double dx, double dy,
+                                double dz, int32_t sigma, int32_t kernel_width,
+                                const anufft_opts *base, int64_t num_points,
+                                const float *const coords[3], int64_t coord_stride,
+                                int32_t workload, const char *cache_path, anufft_opts *tuned,
+                                double *best_ms);
 
 /* Host-pointer variants for drop-in parity tests: copy in, execute on an
  * internal stream of the plan's device, copy out, synchronize.  They allocate

@@ -282,3 +282,30 @@ def test_fp32_cell_division_is_exact():
     for b in range(1, 8192):
         q = (cf * (np.float32(1.0) / np.float32(b))).astype(np.int64)
         assert np.array_equal(q, cells // b), b
+
+
+def test_launch_settings_and_autotune_argument_checks():
+    """Launch settings are validated at plan creation and reported by the
+    metadata; anufft_autotune3d rejects bad arguments before device work."""
+    import ctypes
+    p = A.NufftPlan3D(16, 16, 16, 0.1, 0.1, 0.1, 2, 6, sub_work=16, blocks_per_cu=3, fft_lines=4)
+    m = p.meta
+    assert (m["sub_work"], m["blocks_per_cu"], m["fft_lines"]) == (16, 3, 4)
+    assert A.NufftPlan3D(16, 16, 16, 0.1, 0.1, 0.1).meta["sub_work"] == 48  # automatic
+    for kw in ({"sub_work": -1}, {"blocks_per_cu": 17}, {"fft_lines": 5000}):
+        with pytest.raises(A.InvalidArgument, match="launch settings"):
+            A.NufftPlan3D(16, 16, 16, 0.1, 0.1, 0.1, **kw)
+    lib = N.lib()
+    o = N.default_opts()
+    out = N.Opts()
+    ms = ctypes.c_double()
+    args = (16, 16, 16, 0.1, 0.1, 0.1, 2, 6, ctypes.byref(o))
+    assert lib.anufft_autotune3d(*args, 1000, None, 0, 3, None, None, None) == N.INVALID_ARG
+    assert lib.anufft_autotune3d(*args, 0, None, 0, 3, None, ctypes.byref(out), None) == N.INVALID_ARG
+    assert lib.anufft_autotune3d(*args, 1000, None, 0, 4, None, ctypes.byref(out), None) == N.INVALID_ARG
+    assert lib.anufft_autotune3d(0, 16, 16, 0.1, 0.1, 0.1, 2, 6, ctypes.byref(o), 1000, None, 0, 3,
+                                 None, ctypes.byref(out), None) == N.INVALID_ARG
+    assert "nx" in lib.anufft_last_error().decode()
+    bad = N.ptr_array([None] * 3)
+    assert lib.anufft_autotune3d(*args, 1000, bad, 3, 3, None, ctypes.byref(out),
+                                 ctypes.byref(ms)) == N.INVALID_ARG
