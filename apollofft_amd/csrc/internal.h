@@ -228,15 +228,17 @@ hipError_t launch_direct_f64(int type, const double *x, const double *y, const d
                              int64_t stride, int64_t num_points, int dim, const int64_t n[3],
                              const double L[3], const double2 *in, double2 *out, hipStream_t s);
 
-hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t num_points,
-                           const float2 *values, float2 *grid, const Geom &g,
-                           const KbPoly &poly, hipStream_t s);
-hipError_t launch_interp1d(int w, const float *x, int64_t stride, int64_t num_points,
-                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+// 1D on binned tiles (records from sort_points with dim 1, values = the
+// point values for spreading, u for the gradient): spreading adds to grid;
+// interpolation writes out (gx NULL) or the coordinate gradient gx.
+hipError_t launch_spread1d(int w, const PointArgs &pa, ZRec zr, float2 *grid, const Geom &g,
+                           const KbPoly &poly, int num_blocks, hipStream_t s);
+hipError_t launch_interp1d(int w, const PointArgs &pa, ZRec zr, const float2 *grid, float2 *out,
+                           float *gx, const Geom &g, const KbPoly &poly, int num_blocks,
                            hipStream_t s);
-hipError_t launch_grad1d(int w, const float *x, int64_t stride, int64_t num_points,
-                         const float2 *grid, const float2 *u, float *gx, const Geom &g,
-                         const KbPoly &poly, hipStream_t s);
+// Interior (cells) of the 1D tiles: a padded tile is one wave of 64 cells.
+constexpr int kTile1d = 64;
+inline int bin1d(int w) { return kTile1d - 2 * w + 1; }
 
 // LDS bytes of the 3D spread/interp kernels for a geometry.
 size_t spread_lds_bytes(int w, int method, const Geom &g);

@@ -259,7 +259,7 @@ Layout layout(const anufft_plan *p, int64_t M) {
     off += align_up(bytes);
     return o;
   };
-  if (p->dim == 3) {
+  {
     int64_t nch = M > 0 ? (M + kSortChunk - 1) / kSortChunk : 1;
     l.nchunks = (int)nch;
     size_t Mp = (size_t)std::max<int64_t>(M, 1);
@@ -279,6 +279,13 @@ Layout layout(const anufft_plan *p, int64_t M) {
     for (const anufft_plan::Binning *b : {&p->bs, &p->bi, &p->bg})
       nsub = std::max<int64_t>(nsub, b->nbins + (M + b->max_sub - 1) / b->max_sub + 1);
     l.subs = take(16 * (size_t)nsub);
+    if (p->dim == 1) {
+      l.zrec = take(32 * Mp);   // point records of the 1D binning
+      l.zrec2 = take(32 * Mp);
+      l.grid = take(8 * (size_t)p->m[0]);
+      l.total = off;
+      return l;
+    }
     if (p->kernel_method == kSpreadMfma) {
       l.zrec = take(32 * Mp);   // ZRec: 32-byte record per point
       l.zrec2 = take(32 * Mp);  // the other record buffer of the binning passes
@@ -289,8 +296,6 @@ Layout layout(const anufft_plan *p, int64_t M) {
     l.xo = take(Mp);
     l.grid = take(8 * (size_t)p->cells());
     l.fftbuf = take(8 * (size_t)(p->m[0] * p->m[1] * p->n[2]));
-  } else {
-    l.grid = take(8 * (size_t)p->m[0]);
   }
   l.total = off;
   return l;
@@ -491,11 +496,13 @@ void set_geom(anufft_plan::Binning &b, const anufft_plan *p, const int bin[3]) {
 
 void choose_bins(anufft_plan *p) {
   if (p->dim == 1) {
-    const int b[3] = {(int)p->m[0], 1, 1};
+    // tiles of one wave (64 cells with their taps); subproblems of at most
+    // 1024 points (one workgroup walks a subproblem 64 points at a time)
+    const int b[3] = {bin1d(p->w), 1, 1};
     set_geom(p->bs, p, b);
-    set_geom(p->bi, p, b);
-    set_geom(p->bg, p, b);
-    p->bs.nbins = p->bi.nbins = p->bg.nbins = 1;
+    p->bs.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 1024;
+    p->bi = p->bs;
+    p->bg = p->bs;
     return;
   }
   const bool user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
@@ -703,9 +710,9 @@ struct Exec {
   hipError_t bin(const anufft_plan::Binning &b, const PointArgs &pa, const float2 *values,
                  bool records) const {
     return timed(ANUFFT_STAGE_BIN, [&] {
-      if (!records) return sort_points(pa, b.g, 3, sort_buffers(b), s);
+      if (!records) return sort_points(pa, b.g, p->dim, sort_buffers(b), s);
       const RecordOut ro{zrec(pa.num_points), zrec(pa.num_points, true), values};
-      return sort_points(pa, b.g, 3, sort_buffers(b), s, &ro);
+      return sort_points(pa, b.g, p->dim, sort_buffers(b), s, &ro);
     });
   }
   // Full weight records of sorted points for the vector-ALU spreading kernel.
@@ -1068,27 +1075,28 @@ anufft_status anufft_execute_type1(const anufft_plan *p, int64_t M, const float 
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
   hipError_t e = hipSuccess;
-  if (M > 0 && p->dim == 3) {
+  if (M > 0) {
     PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+    const bool records = p->dim == 1 || p->kernel_method == kSpreadMfma;
     // zeroing the grid (second stream) is independent of the binning
     e = ex.fork_join(
         [&](const Exec &ea) {
           return ea.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ea.grid(), p->cells(), ea.s); });
         },
-        [&] { return ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma); });
+        [&] { return ex.bin(p->bs, pa, (const float2 *)values, records); });
     if (e != hipSuccess) return hip_fail(e, "fill / binning");
-    e = ex.spread3(pa, (const float2 *)values, ex.grid());
+    if (p->dim == 3) {
+      e = ex.spread3(pa, (const float2 *)values, ex.grid());
+    } else {
+      e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
+        return launch_spread1d(p->w, pa, ex.zrec(M), ex.grid(), p->bs.g, p->poly,
+                               ex.blocks(p->bs, M, 8), ex.s);
+      });
+    }
     if (e != hipSuccess) return hip_fail(e, "spread");
   } else {
     e = ex.timed(ANUFFT_STAGE_FILL, [&] { return launch_fill(ex.grid(), p->cells(), ex.s); });
     if (e != hipSuccess) return hip_fail(e, "fill");
-  }
-  if (M > 0 && p->dim != 3) {
-    e = ex.timed(ANUFFT_STAGE_SPREAD, [&] {
-      return launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, ex.grid(),
-                             p->bs.g, p->poly, ex.s);
-    });
-    if (e != hipSuccess) return hip_fail(e, "spread");
   }
   e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.grid_to_modes((float2 *)out); });
   if (e != hipSuccess) return hip_fail(e, "fft");
@@ -1157,11 +1165,16 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
                              ex.blocks(p->bi, M, p->kernel_method == kSpreadMfma ? 2 : 4), ex.s);
     });
   } else {
-    e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes, false); });
-    if (e != hipSuccess) return hip_fail(e, "fft");
+    PointArgs pa = ex.point_args(p->bi, M, coords, stride);
+    e = ex.fork_join(
+        [&](const Exec &ea) {
+          return ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)modes, false); });
+        },
+        [&] { return ex.bin(p->bi, pa, nullptr, true); });
+    if (e != hipSuccess) return hip_fail(e, "fft / binning");
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
-      return launch_interp1d(p->w, coords[0], stride, M, ex.grid(), (float2 *)out, p->bi.g,
-                             p->poly, ex.s);
+      return launch_interp1d(p->w, pa, ex.zrec(M), ex.grid(), (float2 *)out, nullptr, p->bi.g,
+                             p->poly, ex.blocks(p->bi, M, 8), ex.s);
     });
   }
   if (e != hipSuccess) return hip_fail(e, "interp");
@@ -1199,11 +1212,17 @@ anufft_status anufft_execute_grad_points(const anufft_plan *p, int64_t M,
                            ex.blocks(b, M, gm ? 1 : 4), ex.s);
     });
   } else {
-    e = ex.timed(ANUFFT_STAGE_FFT, [&] { return ex.modes_to_grid((const float2 *)modes_h, false); });
-    if (e != hipSuccess) return hip_fail(e, "fft");
+    // records carry u (value fields): the gradient reads it in sorted order
+    PointArgs pa = ex.point_args(p->bg, M, coords, stride);
+    e = ex.fork_join(
+        [&](const Exec &ea) {
+          return ea.timed(ANUFFT_STAGE_FFT, [&] { return ea.modes_to_grid((const float2 *)modes_h, false); });
+        },
+        [&] { return ex.bin(p->bg, pa, (const float2 *)u, true); });
+    if (e != hipSuccess) return hip_fail(e, "fft / binning");
     e = ex.timed(ANUFFT_STAGE_GRAD, [&] {
-      return launch_grad1d(p->w, coords[0], stride, M, ex.grid(), (const float2 *)u, grad[0],
-                           p->bi.g, p->poly, ex.s);
+      return launch_interp1d(p->w, pa, ex.zrec(M), ex.grid(), nullptr, grad[0], p->bg.g, p->poly,
+                             ex.blocks(p->bg, M, 8), ex.s);
     });
   }
   if (e != hipSuccess) return hip_fail(e, "grad");
@@ -1388,12 +1407,11 @@ anufft_status anufft_debug_sort(const anufft_plan *p, int64_t M, const float *co
   Layout lay;
   anufft_status st = check_exec(p, M, coords, stride, perm, bin_start, ws, ws_bytes, lay);
   if (st != ANUFFT_OK) return st;
-  if (p->dim != 3) return fail(ANUFFT_UNSUPPORTED, "1D plans do not bin points");
   DeviceGuard guard(p->device);
   Exec ex{p, (hipStream_t)stream, (char *)ws, lay};
   PointArgs pa = ex.point_args(p->bs, M, coords, stride);
   SortBuffers sb = ex.sort_buffers(p->bs);
-  hipError_t e = sort_points(pa, p->bs.g, 3, sb, ex.s);
+  hipError_t e = sort_points(pa, p->bs.g, p->dim, sb, ex.s);
   if (e != hipSuccess) return hip_fail(e, "binning");
   if (M > 0 && (e = hipMemcpyAsync(perm, pa.perm, 4 * (size_t)M, hipMemcpyDeviceToDevice, ex.s)) != hipSuccess)
     return hip_fail(e, "copy perm");
@@ -1413,15 +1431,14 @@ anufft_status anufft_debug_spread(const anufft_plan *p, int64_t M, const float *
   hipError_t e = launch_fill((float2 *)grid, p->cells(), ex.s);
   if (e != hipSuccess) return hip_fail(e, "fill");
   if (M == 0) return ANUFFT_OK;
-  if (p->dim == 3) {
-    PointArgs pa = ex.point_args(p->bs, M, coords, stride);
-    e = ex.bin(p->bs, pa, (const float2 *)values, p->kernel_method == kSpreadMfma);
-    if (e != hipSuccess) return hip_fail(e, "binning");
+  PointArgs pa = ex.point_args(p->bs, M, coords, stride);
+  e = ex.bin(p->bs, pa, (const float2 *)values, p->dim == 1 || p->kernel_method == kSpreadMfma);
+  if (e != hipSuccess) return hip_fail(e, "binning");
+  if (p->dim == 3)
     e = ex.spread3(pa, (const float2 *)values, (float2 *)grid);
-  } else {
-    e = launch_spread1d(p->w, coords[0], stride, M, (const float2 *)values, (float2 *)grid,
-                        p->bs.g, p->poly, ex.s);
-  }
+  else
+    e = launch_spread1d(p->w, pa, ex.zrec(M), (float2 *)grid, p->bs.g, p->poly,
+                        ex.blocks(p->bs, M, 8), ex.s);
   if (e != hipSuccess) return hip_fail(e, "spread");
   return ANUFFT_OK;
 }

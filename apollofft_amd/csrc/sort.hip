@@ -431,6 +431,7 @@ __device__ __forceinline__ uint32_t fold_point(const float (&c)[3], const Geom &
                                                float (&z)[3], int &off) {
   int cell[3] = {0, 0, 0};
   off = 0;
+  z[0] = z[1] = z[2] = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (a > 0 && dim != 3) break;

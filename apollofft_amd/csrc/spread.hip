@@ -1651,74 +1651,133 @@ __global__ __launch_bounds__(kInterpThreads) void grad3d_kernel(PointArgs pa,
 }
 
 // ---- 1D: one thread per point, global atomics / gathers ---------------------
+// ---- 1D: binned tiles of one wave --------------------------------------------
+//
+// Replaces the 1D loops of apollo-nufft (type1_into spread, transform/
+// dimension_1d.rs:186-197; type2_into gather, :261-275) after the points are
+// binned (sort_points_1d / sort_positions_1d, :54-99, replaced by the
+// counting sort on 1D tiles).  The tile of bin b covers k1Tile cells from
+// x0 = b bin - W + 1, with bin = k1Tile - 2W + 1, so every tap of its points
+// lies inside.  A 64-thread workgroup takes subproblems of the list in turn
+// (grid stride); lane j owns tile cell j.  Points come from the records of
+// the binning (polynomial argument z, cell offset in the tile, value or u,
+// input index).
+constexpr int k1Tile = kTile1d;
+
+// The 2W tap weights (and derivatives) at polynomial argument z, with
+// kb_taps's endpoint rule.
 template <int W>
-__global__ void spread1d_kernel(const float *x, int64_t stride, int64_t n,
-                                const float2 *values, float2 *grid, Geom g, KbPoly poly) {
-  constexpr int T = 2 * W;
-  const int m = g.m[0];
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    int cell;
-    double t = fold_coord(x[j * stride], g.scale[0], m, g.inv_m[0], cell);
-    float wt[T];
-    int i0 = kb_taps<W>(t, cell, poly, wt);
-    const float2 c = values[j];
+__device__ __forceinline__ void taps_z(const KbPoly &P, float z, float (&wt)[2 * W]) {
 #pragma unroll
-    for (int i = 0; i < T; ++i) {
-      int idx = (i0 + i) % m;
-      idx = idx < 0 ? idx + m : idx;
-      unsafeAtomicAdd(&grid[idx].x, c.x * wt[i]);
-      unsafeAtomicAdd(&grid[idx].y, c.y * wt[i]);
-    }
+  for (int i = 0; i < 2 * W; ++i) {
+    float v = P.c[kPolyCoefs - 1][i];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, P.c[d][i]);
+    wt[i] = v;
   }
+  if (z <= -1.0f) wt[0] = 0.0f;
+  if (z >= 1.0f) wt[2 * W - 1] = 0.0f;
 }
 
 template <int W>
-__global__ void interp1d_kernel(const float *x, int64_t stride, int64_t n, const float2 *grid,
-                                float2 *out, Geom g, KbPoly poly) {
+__device__ __forceinline__ void taps_z_deriv(const KbPoly &P, float z, float (&dwt)[2 * W]) {
+#pragma unroll
+  for (int i = 0; i < 2 * W; ++i) {
+    float v = P.dc[kPolyCoefs - 1][i];
+#pragma unroll
+    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, P.dc[d][i]);
+    dwt[i] = v;
+  }
+  if (z <= -1.0f) dwt[0] = 0.0f;
+  if (z >= 1.0f) dwt[2 * W - 1] = 0.0f;
+}
+
+// Spreading: the subproblem's points are staged 64 at a time (weights,
+// values, offsets in LDS); each lane then adds, in list order, the
+// contributions of the staged points whose taps cover its cell, and the
+// finished tile is added to the grid (cells shared with neighbouring tiles
+// by atomics).
+template <int W>
+__global__ __launch_bounds__(k1Tile) void spread1d_binned_kernel(PointArgs pa, ZRec zr,
+                                                                 float2 *grid, Geom g,
+                                                                 KbPoly poly) {
   constexpr int T = 2 * W;
-  const int m = g.m[0];
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    int cell;
-    double t = fold_coord(x[j * stride], g.scale[0], m, g.inv_m[0], cell);
-    float wt[T];
-    int i0 = kb_taps<W>(t, cell, poly, wt);
+  __shared__ float wts[k1Tile][T + 1];
+  __shared__ float2 cv[k1Tile];
+  __shared__ int ofs[k1Tile];
+  const int lane = threadIdx.x, m = g.m[0];
+  const uint32_t nsub = pa.nsub[0];
+  for (uint32_t si = blockIdx.x; si < nsub; si += gridDim.x) {
+    const int4 sub = pa.subs[si];
+    const int x0 = sub.x * g.bin[0] - W + 1;
     float ar = 0.f, ai = 0.f;
+    for (int c0 = 0; c0 < sub.z; c0 += k1Tile) {
+      const int np = min(k1Tile, sub.z - c0);
+      __syncthreads();  // the previous batch is consumed
+      if (lane < np) {
+        const int64_t r = (int64_t)sub.y + c0 + lane;
+        const float4 ra = zr.r[2 * r], rb = zr.r[2 * r + 1];
+        float wt[T];
+        taps_z<W>(poly, ra.x, wt);
 #pragma unroll
-    for (int i = 0; i < T; ++i) {
-      int idx = (i0 + i) % m;
-      idx = idx < 0 ? idx + m : idx;
-      const float2 v = grid[idx];
-      ar = fmaf(wt[i], v.x, ar);
-      ai = fmaf(wt[i], v.y, ai);
+        for (int i = 0; i < T; ++i) wts[lane][i] = wt[i];
+        cv[lane] = make_float2(rb.x, rb.y);
+        ofs[lane] = __float_as_int(ra.w) & 255;
+      }
+      __syncthreads();
+      for (int q = 0; q < np; ++q) {
+        const int d = lane - ofs[q];
+        if (d >= 0 && d < T) {
+          const float wv = wts[q][d];
+          const float2 c = cv[q];
+          ar = fmaf(c.x, wv, ar);
+          ai = fmaf(c.y, wv, ai);
+        }
+      }
     }
-    out[j] = make_float2(ar, ai);
+    if (ar != 0.f || ai != 0.f) {
+      const int x = wrap(x0 + lane, m);
+      unsafeAtomicAdd(&grid[x].x, ar);
+      unsafeAtomicAdd(&grid[x].y, ai);
+    }
   }
 }
 
-template <int W>
-__global__ void grad1d_kernel(const float *x, int64_t stride, int64_t n, const float2 *grid,
-                              const float2 *u, float *gx, Geom g, KbPoly poly) {
+// Interpolation (DERIV: the coordinate gradient): the subproblem's tile of
+// the grid in LDS, then lane p takes point p of each batch of 64.
+template <int W, bool DERIV>
+__global__ __launch_bounds__(k1Tile) void interp1d_binned_kernel(PointArgs pa, ZRec zr,
+                                                                 const float2 *grid, float2 *out,
+                                                                 float *gx, Geom g,
+                                                                 KbPoly poly) {
   constexpr int T = 2 * W;
-  const int m = g.m[0];
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    int cell;
-    double t = fold_coord(x[j * stride], g.scale[0], m, g.inv_m[0], cell);
-    float wt[T], dwt[T];
-    int i0 = kb_taps_deriv<W>(t, cell, poly, wt, dwt);
-    float dr = 0.f, di = 0.f;
+  __shared__ float2 tile[k1Tile];
+  const int lane = threadIdx.x, m = g.m[0];
+  const uint32_t nsub = pa.nsub[0];
+  for (uint32_t si = blockIdx.x; si < nsub; si += gridDim.x) {
+    const int4 sub = pa.subs[si];
+    const int x0 = sub.x * g.bin[0] - W + 1;
+    __syncthreads();  // the previous tile is consumed
+    tile[lane] = grid[wrap(x0 + lane, m)];
+    __syncthreads();
+    for (int c0 = lane; c0 < sub.z; c0 += k1Tile) {
+      const int64_t r = (int64_t)sub.y + c0;
+      const float4 ra = zr.r[2 * r], rb = zr.r[2 * r + 1];
+      const int o = __float_as_int(ra.w) & 255;
+      float wt[T];
+      if (DERIV) taps_z_deriv<W>(poly, ra.x, wt);
+      else taps_z<W>(poly, ra.x, wt);
+      float sr = 0.f, si2 = 0.f;
 #pragma unroll
-    for (int i = 0; i < T; ++i) {
-      int idx = (i0 + i) % m;
-      idx = idx < 0 ? idx + m : idx;
-      const float2 v = grid[idx];
-      dr = fmaf(dwt[i], v.x, dr);
-      di = fmaf(dwt[i], v.y, di);
+      for (int i = 0; i < T; ++i) {
+        const float2 v = tile[o + i];
+        sr = fmaf(wt[i], v.x, sr);
+        si2 = fmaf(wt[i], v.y, si2);
+      }
+      const uint32_t j = __float_as_uint(rb.w);
+      if (DERIV) gx[j] = -(float)g.scale[0] * (rb.x * sr + rb.y * si2);
+      else out[j] = make_float2(sr, si2);
     }
-    const float2 uj = u[j];
-    gx[j] = -(float)g.scale[0] * (uj.x * dr + uj.y * di);
   }
 }
 
@@ -1921,40 +1980,26 @@ struct Grad3 {
   }
 };
 
-inline unsigned blocks_1d(int64_t n) {
-  int64_t b = (n + 255) / 256;
-  if (b < 1) b = 1;
-  if (b > 65536) b = 65536;
-  return (unsigned)b;
-}
-
 template <int W>
 struct Spread1 {
-  static hipError_t run(const float *x, int64_t stride, int64_t n, const float2 *values,
-                        float2 *grid, const Geom &g, const KbPoly &poly, hipStream_t s) {
-    hipLaunchKernelGGL(spread1d_kernel<W>, dim3(blocks_1d(n)), dim3(256), 0, s, x, stride, n,
-                       values, grid, g, poly);
+  static hipError_t run(const PointArgs &pa, ZRec zr, float2 *grid, const Geom &g,
+                        const KbPoly &poly, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(spread1d_binned_kernel<W>, dim3(blocks), dim3(k1Tile), 0, s, pa, zr, grid,
+                       g, poly);
     return hipGetLastError();
   }
 };
 
 template <int W>
 struct Interp1 {
-  static hipError_t run(const float *x, int64_t stride, int64_t n, const float2 *grid,
-                        float2 *out, const Geom &g, const KbPoly &poly, hipStream_t s) {
-    hipLaunchKernelGGL(interp1d_kernel<W>, dim3(blocks_1d(n)), dim3(256), 0, s, x, stride, n,
-                       grid, out, g, poly);
-    return hipGetLastError();
-  }
-};
-
-template <int W>
-struct Grad1 {
-  static hipError_t run(const float *x, int64_t stride, int64_t n, const float2 *grid,
-                        const float2 *u, float *gx, const Geom &g, const KbPoly &poly,
-                        hipStream_t s) {
-    hipLaunchKernelGGL(grad1d_kernel<W>, dim3(blocks_1d(n)), dim3(256), 0, s, x, stride, n,
-                       grid, u, gx, g, poly);
+  static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float2 *out,
+                        float *gx, const Geom &g, const KbPoly &poly, int blocks, hipStream_t s) {
+    if (gx)
+      hipLaunchKernelGGL((interp1d_binned_kernel<W, true>), dim3(blocks), dim3(k1Tile), 0, s, pa,
+                         zr, grid, out, gx, g, poly);
+    else
+      hipLaunchKernelGGL((interp1d_binned_kernel<W, false>), dim3(blocks), dim3(k1Tile), 0, s, pa,
+                         zr, grid, out, gx, g, poly);
     return hipGetLastError();
   }
 };
@@ -2034,20 +2079,14 @@ hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const 
     return dispatch_w<Grad3Mfma>(w, pa, zr, grid, u, gx, gy, gz, g, dpoly, nblk, s);
   return dispatch_w<Grad3>(w, pa, grid, u, gx, gy, gz, g, poly, nblk, s);
 }
-hipError_t launch_spread1d(int w, const float *x, int64_t stride, int64_t n,
-                           const float2 *values, float2 *grid, const Geom &g,
-                           const KbPoly &poly, hipStream_t s) {
-  return dispatch_w<Spread1>(w, x, stride, n, values, grid, g, poly, s);
+hipError_t launch_spread1d(int w, const PointArgs &pa, ZRec zr, float2 *grid, const Geom &g,
+                           const KbPoly &poly, int num_blocks, hipStream_t s) {
+  return dispatch_w<Spread1>(w, pa, zr, grid, g, poly, num_blocks, s);
 }
-hipError_t launch_interp1d(int w, const float *x, int64_t stride, int64_t n,
-                           const float2 *grid, float2 *out, const Geom &g, const KbPoly &poly,
+hipError_t launch_interp1d(int w, const PointArgs &pa, ZRec zr, const float2 *grid, float2 *out,
+                           float *gx, const Geom &g, const KbPoly &poly, int num_blocks,
                            hipStream_t s) {
-  return dispatch_w<Interp1>(w, x, stride, n, grid, out, g, poly, s);
-}
-hipError_t launch_grad1d(int w, const float *x, int64_t stride, int64_t n, const float2 *grid,
-                         const float2 *u, float *gx, const Geom &g, const KbPoly &poly,
-                         hipStream_t s) {
-  return dispatch_w<Grad1>(w, x, stride, n, grid, u, gx, g, poly, s);
+  return dispatch_w<Interp1>(w, pa, zr, grid, out, gx, g, poly, num_blocks, s);
 }
 size_t spread_lds_bytes(int w, int method, const Geom &g) {
   if (method == kSpreadMfma) {
