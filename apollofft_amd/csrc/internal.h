@@ -159,6 +159,24 @@ struct RecordOut {
   const float2 *values;  // point values (Type-1) or null
 };
 
+// One problem of the one-pass binning (sort.hip).  A single binning passes
+// it by value; the batched binning reads a device table of them, chunk0 /
+// scan0 being the problem's first chunk / scan block in the shared launches.
+struct BinProblem {
+  PointArgs pa;             // coordinates and point count
+  Geom g;
+  uint32_t *cnt;            // per-chunk tile counts, then offsets
+  uint32_t *tot;            // tile totals
+  uint32_t *bin_start;      // nbins + 1
+  uint32_t *nsub;           // [0] subproblems, [1] their total work
+  int4 *subs;
+  ZRec out;                 // records (REC) ...
+  uint32_t *keys_out, *idx_out;  // ... or sorted keys and the permutation
+  const float2 *values;     // record values (Type-1 strengths, gradient u) or null
+  int dim, nbins, bits, max_sub, sub_work, nchunks;
+  uint32_t chunk0, scan0;
+};
+
 // One problem of a batched spreading launch.
 struct SpreadProblem {
   PointArgs pa;
@@ -181,6 +199,12 @@ hipError_t launch_fill(float2 *p, int64_t n, hipStream_t s);
 
 hipError_t sort_points(const PointArgs &pa, const Geom &g, int dim, const SortBuffers &sb,
                        hipStream_t s, const RecordOut *rec = nullptr);
+// Binning of nprob problems in shared launches (count, scan, subproblem
+// lists, scatter: four launches for all), records written to each problem's
+// out; every problem must take the one-pass binning.  host_probs' chunk0,
+// scan0, bits and nchunks are filled here; dev_table holds nprob entries.
+hipError_t sort_points_batched(BinProblem *host_probs, int nprob, BinProblem *dev_table,
+                               hipStream_t s);
 // Whether a tile count takes the one-pass binning, and the entries of
 // SortBuffers::cnt the binning of num_points points needs.
 bool sort_one_pass(int nbins);

@@ -1455,6 +1455,7 @@ anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32
   }
   tot += align_up(sizeof(SpreadProblem) * (size_t)std::max(nprob, 1));
   tot += align_up(sizeof(uint32_t) * (size_t)(2 * nprob + 2));  // subproblem and work prefixes
+  tot += align_up(sizeof(BinProblem) * (size_t)std::max(nprob, 1));  // fused binning table
   *bytes = tot;
   return ANUFFT_OK;
 }
@@ -1523,6 +1524,8 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
   SpreadProblem *table = (SpreadProblem *)((char *)ws + off);
   off += align_up(sizeof(SpreadProblem) * (size_t)nprob);
   uint32_t *prefix = (uint32_t *)((char *)ws + off);
+  off += align_up(sizeof(uint32_t) * (size_t)(2 * nprob + 2));
+  BinProblem *bin_table = (BinProblem *)((char *)ws + off);
   hipError_t e;
   const Exec &e0 = ex[0];
   const bool mf = p0->kernel_method == kSpreadMfma;
@@ -1547,6 +1550,31 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
     }
     return hipSuccess;
   };
+  // all problems with one-pass tile counts and records: one fused binning
+  // (four launches for all problems) beside the grids' zeroing
+  bool one_launch = mf;
+  for (int i = 0; i < nprob && one_launch; ++i) one_launch = sort_one_pass(plans[i]->bs.nbins);
+  std::vector<BinProblem> bprobs;
+  if (one_launch) {
+    bprobs.resize(nprob);
+    for (int i = 0; i < nprob; ++i) {
+      const SortBuffers sb = ex[i].sort_buffers(plans[i]->bs);
+      BinProblem &b = bprobs[i];
+      b.pa = probs[i].pa;
+      b.g = plans[i]->bs.g;
+      b.cnt = sb.cnt;
+      b.tot = sb.scan_tmp;
+      b.bin_start = sb.bin_start;
+      b.nsub = sb.sub_ofs + sb.nbins;
+      b.subs = sb.subs;
+      b.out = probs[i].zr;
+      b.values = (const float2 *)values[i];
+      b.dim = 3;
+      b.nbins = sb.nbins;
+      b.max_sub = std::max(sb.max_sub, kBatchMinSub);
+      b.sub_work = sub_work;
+    }
+  }
   e = e0.timed(ANUFFT_STAGE_BIN, [&] {
     return e0.fork_join(
         [&](const Exec &ea) {
@@ -1554,9 +1582,12 @@ anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int3
             hipError_t r = launch_fill(ex[i].grid(), plans[i]->cells(), ea.s);
             if (r != hipSuccess) return r;
           }
-          return sort_range(1, ea.s);
+          return one_launch ? hipSuccess : sort_range(1, ea.s);
         },
-        [&] { return sort_range(0, s); });
+        [&] {
+          return one_launch ? sort_points_batched(bprobs.data(), nprob, bin_table, s)
+                            : sort_range(0, s);
+        });
   });
   if (e != hipSuccess) return hip_fail(e, "fill / binning");
   if (!mf) {

@@ -301,11 +301,9 @@ __device__ __forceinline__ uint64_t block_scan_waves64(uint64_t v, uint64_t *sh,
 constexpr int kSubThreads = 1024;
 constexpr int kSubPer = 4;
 constexpr int kSubRound = kSubThreads * kSubPer;
-__global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_start,
-                                                                  const uint32_t *tot, int from_tot,
-                                                                  int nbins, int3 nb, int max_sub,
-                                                                  int sub_work, uint32_t *nsub,
-                                                                  int4 *subs) {
+__device__ __forceinline__ void subproblems_body(uint32_t *bin_start, const uint32_t *tot,
+                                                 int from_tot, int nbins, int3 nb, int max_sub,
+                                                 int sub_work, uint32_t *nsub, int4 *subs) {
   __shared__ uint32_t sh[kSubThreads / 64];
   __shared__ uint64_t sh64[kSubThreads / 64];
   __shared__ uint32_t first[kSubRound];  // first entry of each tile of the round
@@ -388,6 +386,14 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_
   }
 }
 
+__global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_start,
+                                                                  const uint32_t *tot, int from_tot,
+                                                                  int nbins, int3 nb, int max_sub,
+                                                                  int sub_work, uint32_t *nsub,
+                                                                  int4 *subs) {
+  subproblems_body(bin_start, tot, from_tot, nbins, nb, max_sub, sub_work, nsub, subs);
+}
+
 // ---- one-pass binning (tile count <= kOneMaxBins) ---------------------------
 // A stable counting sort on the whole tile key in one scatter: chunks of
 // kOneChunk points in input order, each split into kOneWaves contiguous runs
@@ -455,11 +461,37 @@ __device__ __forceinline__ uint32_t fold_point(const float (&c)[3], const Geom &
 // LDS (the sums do not depend on the order of the atomic adds).
 constexpr int kCountThreads = 1024;
 constexpr int kCountPer = kOneChunk / kCountThreads;
-__global__ __launch_bounds__(kCountThreads) void chunk_count_kernel(PointArgs pa, Geom g, int dim,
-                                                                    int nbins, uint32_t *cnt) {
+// The problem of this block: the one passed by value, or (BATCH) the last
+// table entry whose first block (chunk0 or scan0) is at most blockIdx.x.
+template <bool BATCH, bool SCAN>
+__device__ __forceinline__ const BinProblem &bin_problem(const BinProblem &one,
+                                                          const BinProblem *table, int nprob,
+                                                          uint32_t &local) {
+  if (!BATCH) {
+    local = blockIdx.x;
+    return one;
+  }
+  int lo = 0, hi = nprob - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((SCAN ? table[mid].scan0 : table[mid].chunk0) <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  local = blockIdx.x - (SCAN ? table[lo].scan0 : table[lo].chunk0);
+  return table[lo];
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(kCountThreads) void chunk_count_kernel(BinProblem one,
+                                                                    const BinProblem *table,
+                                                                    int nprob) {
   extern __shared__ uint32_t hist[];  // nbins
+  uint32_t chunk;
+  const BinProblem &pr = bin_problem<BATCH, false>(one, table, nprob, chunk);
+  const int nbins = pr.nbins, dim = pr.dim;
+  const PointArgs &pa = pr.pa;
   for (int d = threadIdx.x; d < nbins; d += kCountThreads) hist[d] = 0;
-  const int64_t base = (int64_t)blockIdx.x * kOneChunk + threadIdx.x;
+  const int64_t base = (int64_t)chunk * kOneChunk + threadIdx.x;
   float c[kCountPer][3];
 #pragma unroll
   for (int s = 0; s < kCountPer; ++s) {
@@ -473,10 +505,10 @@ __global__ __launch_bounds__(kCountThreads) void chunk_count_kernel(PointArgs pa
     const bool valid = j < pa.num_points;
     float z[3];
     int off;
-    if (valid) atomicAdd(&hist[fold_point<false>(c[s], g, dim, z, off)], 1u);
+    if (valid) atomicAdd(&hist[fold_point<false>(c[s], pr.g, dim, z, off)], 1u);
   }
   __syncthreads();
-  uint32_t *row = cnt + (int64_t)blockIdx.x * nbins;
+  uint32_t *row = pr.cnt + (int64_t)chunk * nbins;
   for (int d = threadIdx.x; d < nbins; d += kCountThreads) row[d] = hist[d];
 }
 
@@ -486,13 +518,16 @@ __global__ __launch_bounds__(kCountThreads) void chunk_count_kernel(PointArgs pa
 // so each thread has one short run of loads in flight.
 constexpr int kScanDigits = 16;
 constexpr int kChunkScanSegs = 64;
-__global__ __launch_bounds__(kScanDigits * kChunkScanSegs) void chunk_scan_kernel(uint32_t *cnt,
-                                                                                  int nchunks,
-                                                                                  int nbins,
-                                                                                  uint32_t *tot) {
+template <bool BATCH>
+__global__ __launch_bounds__(kScanDigits * kChunkScanSegs) void chunk_scan_kernel(
+    BinProblem one, const BinProblem *table, int nprob) {
   __shared__ uint32_t seg[kChunkScanSegs][kScanDigits];
+  uint32_t blk;
+  const BinProblem &pr = bin_problem<BATCH, true>(one, table, nprob, blk);
+  uint32_t *cnt = pr.cnt;
+  const int nchunks = pr.nchunks, nbins = pr.nbins;
   const int dl = threadIdx.x & (kScanDigits - 1), sg = threadIdx.x / kScanDigits;
-  const int d = blockIdx.x * kScanDigits + dl;
+  const int d = (int)blk * kScanDigits + dl;
   const int per = (nchunks + kChunkScanSegs - 1) / kChunkScanSegs;
   const int c0 = min(nchunks, sg * per), c1 = min(nchunks, c0 + per);
   uint32_t s = 0;
@@ -517,21 +552,33 @@ __global__ __launch_bounds__(kScanDigits * kChunkScanSegs) void chunk_scan_kerne
       *q = before;
       before += v;
     }
-    if (sg == 0) tot[d] = total;
+    if (sg == 0) pr.tot[d] = total;
   }
 }
 
-template <bool REC>
-__global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(
-    PointArgs pa, Geom g, int dim, int nbins, int bits, const uint32_t *offs,
-    const uint32_t *bin_start, uint32_t *keys_out, uint32_t *idx_out, ZRec out,
-    const float2 *values) {
+// Subproblem lists of a batch: one block per problem.
+__global__ __launch_bounds__(kSubThreads) void subproblems_batched_kernel(const BinProblem *table) {
+  const BinProblem &pr = table[blockIdx.x];
+  const int3 nb = pr.dim == 3 ? make_int3(pr.g.nbin[0], pr.g.nbin[1], pr.g.nbin[2])
+                              : make_int3(1, 1, 1);
+  subproblems_body(pr.bin_start, pr.pa.num_points > 0 ? pr.tot : nullptr, 1, pr.nbins, nb,
+                   pr.max_sub, pr.sub_work, pr.nsub, pr.subs);
+}
+
+template <bool REC, bool BATCH>
+__global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(BinProblem one,
+                                                                    const BinProblem *table,
+                                                                    int nprob) {
   extern __shared__ uint16_t rows[];  // [kOneWaves][nbins]: per-wave tile counts, then offsets
+  uint32_t chunk;
+  const BinProblem &pr = bin_problem<BATCH, false>(one, table, nprob, chunk);
+  const int nbins = pr.nbins, dim = pr.dim, bits = pr.bits;
+  const PointArgs &pa = pr.pa;
   for (int i = threadIdx.x; i < kOneWaves / 2 * nbins; i += kOneThreads) ((uint32_t *)rows)[i] = 0u;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint16_t *row = rows + wave * nbins;
-  const int64_t j0 = (int64_t)blockIdx.x * kOneChunk + (int64_t)wave * (64 * kOneSteps) + lane;
+  const int64_t j0 = (int64_t)chunk * kOneChunk + (int64_t)wave * (64 * kOneSteps) + lane;
   // the wave's coordinates, all loads in flight at once
   float c[kOneSteps][3];
 #pragma unroll
@@ -547,7 +594,7 @@ __global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(
   for (int s = 0; s < kOneSteps; ++s) {
     const int64_t j = j0 + 64 * s;
     const bool valid = j < pa.num_points;
-    const uint32_t k = valid ? fold_point<REC>(c[s], g, dim, zr[s], off[s]) : 0u;
+    const uint32_t k = valid ? fold_point<REC>(c[s], pr.g, dim, zr[s], off[s]) : 0u;
     const uint64_t peers = key_peers(k, valid, bits);
     const uint64_t below = peers & lt;
     uint32_t b0 = 0;
@@ -571,7 +618,8 @@ __global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(
     }
   }
   __syncthreads();
-  const uint32_t *coff = offs + (int64_t)blockIdx.x * nbins;
+  const uint32_t *coff = pr.cnt + (int64_t)chunk * nbins;
+  const uint32_t *bin_start = pr.bin_start;
   uint32_t pos[kOneSteps];
   float2 val[kOneSteps];
 #pragma unroll
@@ -580,21 +628,47 @@ __global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(
     const bool valid = j < pa.num_points;
     const uint32_t k = key[s];
     pos[s] = valid ? bin_start[k] + coff[k] + (uint32_t)row[k] + rk[s] : 0u;
-    val[s] = (REC && values && valid) ? values[j] : make_float2(0.f, 0.f);
+    val[s] = (REC && pr.values && valid) ? pr.values[j] : make_float2(0.f, 0.f);
   }
 #pragma unroll
   for (int s = 0; s < kOneSteps; ++s) {
     const int64_t j = j0 + 64 * s;
     if (j >= pa.num_points) continue;
     if (REC) {
-      out.r[2 * (int64_t)pos[s]] = make_float4(zr[s][0], zr[s][1], zr[s][2], __int_as_float(off[s]));
-      out.r[2 * (int64_t)pos[s] + 1] =
+      pr.out.r[2 * (int64_t)pos[s]] =
+          make_float4(zr[s][0], zr[s][1], zr[s][2], __int_as_float(off[s]));
+      pr.out.r[2 * (int64_t)pos[s] + 1] =
           make_float4(val[s].x, val[s].y, __uint_as_float(key[s]), __uint_as_float((uint32_t)j));
     } else {
-      keys_out[pos[s]] = key[s];
-      idx_out[pos[s]] = (uint32_t)j;
+      pr.keys_out[pos[s]] = key[s];
+      pr.idx_out[pos[s]] = (uint32_t)j;
     }
   }
+}
+
+constexpr int kBinTableChunk = 12;  // descriptors per table-fill launch (kernarg size)
+struct BinTableChunk {
+  int first, count;
+  BinProblem probs[kBinTableChunk];
+};
+__global__ void bin_table_kernel(BinTableChunk chunk, BinProblem *table) {
+  const int i = threadIdx.x;
+  if (i < chunk.count) table[chunk.first + i] = chunk.probs[i];
+}
+
+int key_bits(int nbins) {
+  int bits = 0;
+  while ((1 << bits) < nbins) ++bits;
+  return bits;
+}
+
+hipError_t set_scatter_lds(bool rec, bool batch, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  const void *fn = rec ? (batch ? (const void *)chunk_scatter_kernel<true, true>
+                                : (const void *)chunk_scatter_kernel<true, false>)
+                       : (batch ? (const void *)chunk_scatter_kernel<false, true>
+                                : (const void *)chunk_scatter_kernel<false, false>);
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
 } // namespace
@@ -614,14 +688,33 @@ static hipError_t sort_points_one_pass(const PointArgs &pa, const Geom &g, int d
                                        const RecordOut *rec) {
   const int64_t n = pa.num_points;
   const int nbins = sb.nbins;
-  int bits = 0;
-  while ((1 << bits) < nbins) ++bits;
-  const int nch = (int)((n + kOneChunk - 1) / kOneChunk);
+  BinProblem pr{};
+  pr.pa = pa;
+  pr.g = g;
+  pr.cnt = sb.cnt;
+  pr.tot = sb.scan_tmp;
+  pr.bin_start = sb.bin_start;
+  pr.nsub = sb.sub_ofs + sb.nbins;
+  pr.subs = sb.subs;
+  if (rec) {
+    pr.out = rec->out;
+    pr.values = rec->values;
+  } else {
+    pr.keys_out = sb.keys[sb.passes & 1];
+    pr.idx_out = sb.idx[sb.passes & 1];
+  }
+  pr.dim = dim;
+  pr.nbins = nbins;
+  pr.bits = key_bits(nbins);
+  pr.max_sub = sb.max_sub;
+  pr.sub_work = sb.sub_work;
+  pr.nchunks = (int)((n + kOneChunk - 1) / kOneChunk);
+  const int nch = pr.nchunks;
   if (n > 0) {
-    hipLaunchKernelGGL(chunk_count_kernel, dim3(nch), dim3(kCountThreads), sizeof(uint32_t) * nbins, s, pa,
-                       g, dim, nbins, sb.cnt);
-    hipLaunchKernelGGL(chunk_scan_kernel, dim3((nbins + kScanDigits - 1) / kScanDigits),
-                       dim3(kScanDigits * kChunkScanSegs), 0, s, sb.cnt, nch, nbins, sb.scan_tmp);
+    hipLaunchKernelGGL(chunk_count_kernel<false>, dim3(nch), dim3(kCountThreads),
+                       sizeof(uint32_t) * nbins, s, pr, (const BinProblem *)nullptr, 1);
+    hipLaunchKernelGGL(chunk_scan_kernel<false>, dim3((nbins + kScanDigits - 1) / kScanDigits),
+                       dim3(kScanDigits * kChunkScanSegs), 0, s, pr, (const BinProblem *)nullptr, 1);
   }
   // tile starts and the subproblem list need only the counts
   const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
@@ -630,23 +723,61 @@ static hipError_t sort_points_one_pass(const PointArgs &pa, const Geom &g, int d
                      sb.nbins, nb, sb.max_sub, sb.sub_work, sb.sub_ofs + sb.nbins, sb.subs);
   if (n > 0) {
     const size_t lds = sizeof(uint16_t) * (size_t)kOneWaves * nbins;
-    if (lds > 65536) {
-      const void *fn = rec ? (const void *)chunk_scatter_kernel<true>
-                           : (const void *)chunk_scatter_kernel<false>;
-      hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-    }
-    ZRec out{};
-    if (rec) {
-      out = rec->out;
-      hipLaunchKernelGGL(chunk_scatter_kernel<true>, dim3(nch), dim3(kOneThreads), lds, s, pa, g,
-                         dim, nbins, bits, sb.cnt, sb.bin_start, (uint32_t *)nullptr,
-                         (uint32_t *)nullptr, out, rec->values);
-    } else {
-      hipLaunchKernelGGL(chunk_scatter_kernel<false>, dim3(nch), dim3(kOneThreads), lds, s, pa, g,
-                         dim, nbins, bits, sb.cnt, sb.bin_start, sb.keys[sb.passes & 1],
-                         sb.idx[sb.passes & 1], out, (const float2 *)nullptr);
-    }
+    hipError_t e = set_scatter_lds(rec != nullptr, false, lds);
+    if (e != hipSuccess) return e;
+    if (rec)
+      hipLaunchKernelGGL((chunk_scatter_kernel<true, false>), dim3(nch), dim3(kOneThreads), lds, s,
+                         pr, (const BinProblem *)nullptr, 1);
+    else
+      hipLaunchKernelGGL((chunk_scatter_kernel<false, false>), dim3(nch), dim3(kOneThreads), lds,
+                         s, pr, (const BinProblem *)nullptr, 1);
+  }
+  return hipGetLastError();
+}
+
+hipError_t sort_points_batched(BinProblem *probs, int nprob, BinProblem *table, hipStream_t s) {
+  if (nprob <= 0) return hipSuccess;
+  uint32_t chunks = 0, scans = 0;
+  int max_bins = 1;
+  bool rec = probs[0].out.r != nullptr;
+  for (int i = 0; i < nprob; ++i) {
+    BinProblem &pr = probs[i];
+    if (!sort_one_pass(pr.nbins) || (pr.out.r != nullptr) != rec) return hipErrorInvalidValue;
+    pr.bits = key_bits(pr.nbins);
+    pr.nchunks = (int)((pr.pa.num_points + kOneChunk - 1) / kOneChunk);
+    pr.chunk0 = chunks;
+    pr.scan0 = scans;
+    chunks += (uint32_t)pr.nchunks;
+    if (pr.pa.num_points > 0) scans += (uint32_t)((pr.nbins + kScanDigits - 1) / kScanDigits);
+    max_bins = std::max(max_bins, pr.nbins);
+  }
+  for (int first = 0; first < nprob; first += kBinTableChunk) {
+    BinTableChunk chunk{};
+    chunk.first = first;
+    chunk.count = std::min(kBinTableChunk, nprob - first);
+    for (int i = 0; i < chunk.count; ++i) chunk.probs[i] = probs[first + i];
+    hipLaunchKernelGGL(bin_table_kernel, dim3(1), dim3(64), 0, s, chunk, table);
+  }
+  const BinProblem none{};
+  if (chunks > 0) {
+    hipLaunchKernelGGL(chunk_count_kernel<true>, dim3(chunks), dim3(kCountThreads),
+                       sizeof(uint32_t) * max_bins, s, none, (const BinProblem *)table, nprob);
+    if (scans > 0)
+      hipLaunchKernelGGL(chunk_scan_kernel<true>, dim3(scans), dim3(kScanDigits * kChunkScanSegs),
+                         0, s, none, (const BinProblem *)table, nprob);
+  }
+  hipLaunchKernelGGL(subproblems_batched_kernel, dim3(nprob), dim3(kSubThreads), 0, s,
+                     (const BinProblem *)table);
+  if (chunks > 0) {
+    const size_t lds = sizeof(uint16_t) * (size_t)kOneWaves * max_bins;
+    hipError_t e = set_scatter_lds(rec, true, lds);
+    if (e != hipSuccess) return e;
+    if (rec)
+      hipLaunchKernelGGL((chunk_scatter_kernel<true, true>), dim3(chunks), dim3(kOneThreads), lds,
+                         s, none, (const BinProblem *)table, nprob);
+    else
+      hipLaunchKernelGGL((chunk_scatter_kernel<false, true>), dim3(chunks), dim3(kOneThreads), lds,
+                         s, none, (const BinProblem *)table, nprob);
   }
   return hipGetLastError();
 }
