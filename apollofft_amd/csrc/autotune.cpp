@@ -83,7 +83,9 @@ std::map<std::string, std::string> read_cache(const char *path) {
     const size_t a = line.find('"');
     const size_t b = a == std::string::npos ? a : line.find("\": {", a + 1);
     if (a == std::string::npos || b == std::string::npos) continue;
-    entries[line.substr(a + 1, b - a - 1)] = line.substr(b + 3);
+    std::string v = line.substr(b + 3);
+    while (!v.empty() && (v.back() == ',' || v.back() == ' ' || v.back() == '\r')) v.pop_back();
+    entries[line.substr(a + 1, b - a - 1)] = v;
   }
   return entries;
 }

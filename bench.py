@@ -103,8 +103,15 @@ def load_traffic(kernel):
         return None
 
 
+TUNE_CACHE = None
+
+
 def tune_cache():
-    """A fresh JSON cache file for the autotuner (nothing carries over between runs)."""
+    """The autotuner's JSON cache: --tune-cache PATH (entries found there are
+    reused without timing runs), else a fresh file, so nothing carries over
+    between runs."""
+    if TUNE_CACHE:
+        return TUNE_CACHE
     import tempfile
     return os.path.join(tempfile.mkdtemp(prefix="anufft_tune_"), "tune.json")
 
@@ -491,6 +498,8 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="automatic launch settings instead of the autotuner's")
+    ap.add_argument("--tune-cache", default="",
+                    help="autotuner cache file to read and extend (default: a fresh one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--method", default="auto", choices=["auto", "valu", "mfma"],
                     help="3D spreading / interpolation kernels")
@@ -498,6 +507,8 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    global TUNE_CACHE
+    TUNE_CACHE = args.tune_cache or None
 
     if args.calibrate_copy:
         import torch

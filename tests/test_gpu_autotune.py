@@ -81,6 +81,13 @@ def test_autotune_on_caller_points(dev, tmp_path):
         assert t[k] in vals, (k, t[k])
     doc = json.loads(cache.read_text())
     assert len(doc) == 2 and any("points=caller" in k and "workload=1" in k for k in doc)
+    # more entries: the file stays valid JSON and earlier entries keep their values
+    A.autotune3d(*n, *d, 2, w, positions=pts, workload="type2", cache=cache, grid_policy="exact")
+    A.autotune3d(*n, *d, 2, w, positions=pts, workload="both", cache=cache, grid_policy="exact")
+    doc2 = json.loads(cache.read_text())
+    assert len(doc2) == 4
+    for k, v in doc.items():
+        assert doc2[k] == v
 
 
 @pytest.mark.parametrize("settings", [
