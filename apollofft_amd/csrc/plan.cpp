@@ -1460,10 +1460,12 @@ anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32
   return ANUFFT_OK;
 }
 
-// Batched Type-1: per-problem binning and FFT passes on one stream, and the
-// spreading of every problem in one persistent launch (problems must share
-// kernel_width and sigma, hence the kernel polynomials; otherwise the
-// problems run one after another).
+// Batched Type-1: the binning of all problems in shared launches (beside the
+// grids' zeroing on the second stream), the spreading of every problem in
+// one persistent launch, and per-problem FFT passes split between the two
+// streams (problems must share kernel_width, sigma and the kernel method,
+// hence the kernel polynomials; otherwise the problems run one after
+// another).
 anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int32_t nprob,
                                            const int64_t *num_points, const float *const *coords,
                                            int64_t stride, const float *const *values,

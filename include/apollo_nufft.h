@@ -229,8 +229,12 @@ anufft_status anufft_execute_type1_backward(const anufft_plan *plan, int64_t num
  * coords_p[a] = coords[3p + a] (stride coord_stride), values[p], out[p].  The
  * workspace must hold anufft_batch_workspace_size bytes.  Work is queued on
  * `hip_stream`; parts of it are forked to the second stream of plans[0] and
- * joined back.  Problems with equal kernel width and sigma share one
- * persistent spreading launch; the binning and FFT passes are per problem. */
+ * joined back.  When all plans have the same kernel width, sigma and kernel
+ * method, the problems share the binning launches (one count, scan, list and
+ * scatter launch for all, when every plan has at most 9216 tiles) and one
+ * persistent spreading launch; the FFT passes run per problem, split
+ * between the caller's stream and plans[0]'s second stream.  Other mixes
+ * run one anufft_execute_type1 per problem. */
 anufft_status anufft_batch_workspace_size(const anufft_plan *const *plans, int32_t num_problems,
                                           const int64_t *max_points, size_t *bytes);
 anufft_status anufft_execute_type1_batched(const anufft_plan *const *plans, int32_t num_problems,
