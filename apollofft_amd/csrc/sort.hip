@@ -422,12 +422,18 @@ __device__ __forceinline__ uint64_t key_peers(uint32_t key, bool valid, int bits
   return peers;
 }
 
-// Coordinates of point j (zeros past the end); dim 1 reads x only.
+// Coordinates of point j (zeros past the end; dim 1 keeps x only).  The
+// loads are unconditional, from point 0 past the end (callers launch only
+// with points; 1D point arguments alias y and z to x), so that no branch
+// around a load makes the compiler wait for it there: all loads of a
+// thread stay in flight together.
 __device__ __forceinline__ void load_point(const PointArgs &pa, int dim, int64_t j, bool valid,
                                            float (&c)[3]) {
-  c[0] = valid ? pa.x[j * pa.stride] : 0.f;
-  c[1] = (valid && dim == 3) ? pa.y[j * pa.stride] : 0.f;
-  c[2] = (valid && dim == 3) ? pa.z[j * pa.stride] : 0.f;
+  const int64_t o = (valid ? j : 0) * pa.stride;
+  const float x = pa.x[o], y = pa.y[o], z = pa.z[o];
+  c[0] = valid ? x : 0.f;
+  c[1] = (valid && dim == 3) ? y : 0.f;
+  c[2] = (valid && dim == 3) ? z : 0.f;
 }
 
 // Tile key of a point from its coordinates (tile_key's arithmetic); with
