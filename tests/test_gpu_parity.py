@@ -635,3 +635,36 @@ def test_graph_replays_match_eager(dev, oracle_mod):
             x64 = xyz.astype(np.float64)
             assert rel_l2(o1.cpu().numpy(), oracle_mod.direct_type1_3d(x64, c.astype(np.complex128), n, d)) < TOL
             assert rel_l2(o2.cpu().numpy(), oracle_mod.direct_type2_3d(f.astype(np.complex128), x64, d)) < TOL
+
+
+def test_batched_falls_back_to_per_problem_binning(dev, oracle_mod):
+    """A batch with one plan above the one-pass tile limit (150^3 modes on a
+    300^3 grid, w = 7: 28,900 tiles, LSD binning) bins per problem and still
+    spreads in the shared launch: every output equals its single call and
+    the FP64 direct sum on sampled modes."""
+    import apollofft_amd as A
+    import apollofft_amd._native as N
+    import ctypes
+    import torch
+    rng = np.random.default_rng(41)
+    shapes, Ms = [(150, 150, 150), (16, 16, 16)], [20_000, 3000]
+    plans, pos, vals, host = [], [], [], []
+    for n, M in zip(shapes, Ms):
+        d = [TWO_PI / k for k in n]
+        plans.append(A.NufftPlan3D(*n, *d, 2, 7, grid_policy="exact"))
+        xyz, c = uniform_points(rng, M), cnormal(rng, M)
+        pos.append(_t(xyz, dev))
+        vals.append(_t(c, dev))
+        host.append((xyz, c, n, d))
+    nb = ctypes.c_int64()
+    N.check(N.lib().anufft_debug_num_bins(plans[0].handle, ctypes.byref(nb)))
+    assert nb.value > 9216
+    outs = A.execute_type1_batched(plans, pos, vals)
+    torch.cuda.synchronize()
+    for p, x, v, o, (xyz, c, n, d) in zip(plans, pos, vals, outs, host):
+        single = p.type1(x, v)
+        torch.cuda.synchronize()
+        assert rel_l2(o.cpu().numpy(), single.cpu().numpy()) < 1e-6
+        sel = rng.choice(int(np.prod(n)), size=300, replace=False).astype(np.int64)
+        ref = oracle_mod.direct_type1_3d(xyz.astype(np.float64), c.astype(np.complex128), n, d, sel)
+        assert rel_l2(o.cpu().numpy().ravel()[sel], ref) < TOL
