@@ -396,6 +396,7 @@ __global__ __launch_bounds__(kSpreadThreads) void spread3d_kernel(PointArgs pa, 
 // cell from the staged y and x weights.
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Operands enter the FP16 split multiplied by 2^12 (exact), so that the low
 // halves of operands down to 2^-14 of the largest stay FP16-normal (11
@@ -1227,6 +1228,339 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
   }
 }
 
+// ---- Windowed MFMA interpolation (2w <= 14) --------------------------------
+//
+// A point's 2w tap rows in y start at its tile offset o1 in [1, 32 - 2w] (the
+// binning's record offset), so a 32 x 32 product block uses 2w of its 32 y
+// rows.  Points whose o1 lie in one class of 17 - 2w consecutive offsets all
+// fit one 16-row window of the tile, starting at row origin(c); for such a
+// group of 16 points the y contraction runs on v_mfma_f32_16x16x32_f16 (16
+// window rows x 16 points, K = the tile's 32 z columns): half the matrix work
+// per point of the 32 x 32 form.
+//
+// The tile lives in registers as two 16-row halves: lane (i, g) (i = lane %
+// 16, g = lane / 16) holds rows i and 16 + i of the z columns 8g .. 8g + 7,
+// which is the A-operand layout of the 16 x 16 x 32 product for the window at
+// row 0 or 16.  The window of a class is rotated out of the halves once per
+// class: lane i' of a 16-lane row passes its lower row if i' >= origin, else
+// its upper row, and lane i takes lane (i + origin) mod 16's value.
+//
+// Each piece of at most kJPiece points of a subproblem is listed in LDS by
+// class, each class padded to whole groups of 16 (pad slots compute on zero
+// arguments and write nothing).  The order inside a class does not matter:
+// each point's sum is formed in a fixed order from its own weights.
+template <int W>
+struct MWin {
+  static constexpr int CW = 17 - 2 * W;                    // offsets per class
+  static constexpr int NCLS = (32 - 2 * W + CW - 1) / CW;  // classes of o1 in [1, 32 - 2w]
+  __device__ static int origin(int c) { return min(1 + c * CW, 16); }
+  __device__ static int cls(int o1) { return min((o1 - 1) / CW, NCLS - 1); }
+};
+constexpr int kJNP = 128;      // slots staged per round (8 groups of 16)
+constexpr int kJRowB = 40;     // row stride (halves) of the staged B operands (32 z columns)
+constexpr int kJRowY = 20;     // row stride (floats) of the y weights (16 window rows)
+constexpr int kJRowX = 20;     // of the x weights (16 columns)
+constexpr int kJPiece = 4096;  // points listed by class at once
+struct JLds {
+  static constexpr int BH = 0;                          // half [kJNP][kJRowB]: hi of z weights
+  static constexpr int BL = BH + kJNP * kJRowB * 2;     // half [kJNP][kJRowB]: lo
+  static constexpr int WY = BL + kJNP * kJRowB * 2;     // float [kJNP][kJRowY]
+  static constexpr int WX = WY + kJNP * kJRowY * 4;     // float [kJNP][kJRowX]
+  static constexpr int RED = WX + kJNP * kJRowX * 4;    // float2 [16][kJNP]: per wave, lane row
+  static constexpr int JIDX = RED + 16 * kJNP * 8;      // int [kJNP]
+  static constexpr int ZS = JIDX + kJNP * 4;            // float [3][kJNP]
+  static constexpr int OFS = ZS + 3 * kJNP * 4;         // int [kJNP]
+  static constexpr int CNT = OFS + kJNP * 4;            // int [8] counts, [8] cursors, [8] starts
+  static constexpr int LIST = CNT + 24 * 4;             // u16 [kJPiece + 16 * 8]
+  static constexpr int POLY = LIST + (kJPiece + 128) * 2;  // float [kMaxTaps][kPolyRow]
+  static constexpr int SIZE = POLY + kMaxTaps * kPolyRow * 4;
+};
+static_assert(2 * JLds::SIZE <= 160 * 1024, "windowed interpolation LDS");
+static_assert(kJPiece < 0xFFFF, "list entries are u16 with 0xFFFF for padding");
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+// Window rows origin .. origin + 15 from the two 16-row halves (see above).
+__device__ __forceinline__ half8 window_rows(const half8 &lower, const half8 &upper, int origin,
+                                             int i16, int addr) {
+  const int4v l = __builtin_bit_cast(int4v, lower), u = __builtin_bit_cast(int4v, upper);
+  const bool low = i16 >= origin;
+  int4v r;
+#pragma unroll
+  for (int d = 0; d < 4; ++d) r[d] = __builtin_amdgcn_ds_bpermute(addr, low ? l[d] : u[d]);
+  return __builtin_bit_cast(half8, r);
+}
+
+template <int W>
+__global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_win_kernel(
+    PointArgs pa, ZRec zr, const float2 *grid, const float *tmax, float2 *out, Geom g,
+    const KbPoly *__restrict__ dpoly) {
+  using MS = MSpread<W>;
+  using MW = MWin<W>;
+  constexpr int NT = MS::NT, XW = MS::XW, NW = MS::NW, NCLS = MW::NCLS;
+  static_assert(NW == 4 && XW == 4 && MS::PX == 16 && NCLS <= 8, "windowed interpolation shape");
+  extern __shared__ __align__(16) char shm[];
+  _Float16 *BH = (_Float16 *)(shm + JLds::BH);
+  _Float16 *BL = (_Float16 *)(shm + JLds::BL);
+  float *WY = (float *)(shm + JLds::WY);
+  float *WX = (float *)(shm + JLds::WX);
+  float2 *RED = (float2 *)(shm + JLds::RED);
+  int *JIDX = (int *)(shm + JLds::JIDX);
+  float *ZS = (float *)(shm + JLds::ZS);
+  int *OFS = (int *)(shm + JLds::OFS);
+  int *CNT = (int *)(shm + JLds::CNT);
+  int *CUR = CNT + 8;
+  int *START = CNT + 16;
+  uint16_t *LIST = (uint16_t *)(shm + JLds::LIST);
+  float *PL = (float *)(shm + JLds::POLY);
+  stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first piece's barriers
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i16 = lane & 15, g4 = lane >> 4;
+  const XcdChunk xr(pa);
+  for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
+    const int4 sub = pa.subs[sidx];
+    const int bin = sub.x;
+    const int bz = bin % g.nbin[2];
+    const int by = (bin / g.nbin[2]) % g.nbin[1];
+    const int bx = bin / (g.nbin[2] * g.nbin[1]);
+    const int bx0 = bx * g.bin[0];
+    const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
+    const float mx = tmax[bin];
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);
+    e = max(-120, min(120, e));
+    const float s_in = ldexpf(1.0f, -e) * kOpScale, s_out = ldexpf(1.0f, e) * kProdUnscale;
+
+    // the tile: G[x][y = 16 hf + i16][z = 8 g4 + j] of this wave's XW x cells
+    half8 trh[XW][2], trl[XW][2], tih[XW][2], til[XW][2];
+    const int zc = 8 * g4, z0 = oz + zc;
+#pragma unroll
+    for (int q = 0; q < XW; ++q) {
+      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf) {
+        const int ty = 16 * hf + i16;
+        const bool yok = ty < g.pad[1];
+        const int gy = wrap(oy + ty, g.m[1]);
+        const float2 *rowp = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2];
+        const int64_t e0 = ((int64_t)gx * g.m[1] + gy) * g.m[2] + z0;  // element index
+        float2 v[8];
+        if (yok && zc + 8 <= g.pad[2] && z0 >= 0 && z0 + 8 <= g.m[2] && (e0 & 1) == 0) {
+          const float4 *src = (const float4 *)(rowp + z0);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float4 t = src[j];
+            v[2 * j] = make_float2(t.x, t.y);
+            v[2 * j + 1] = make_float2(t.z, t.w);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            v[j] = (yok && zc + j < g.pad[2]) ? rowp[wrap(z0 + j, g.m[2])] : make_float2(0.f, 0.f);
+        }
+        float tr[8], ti[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          tr[j] = v[j].x * s_in;
+          ti[j] = v[j].y * s_in;
+        }
+        split8(tr, trh[q][hf], trl[q][hf]);
+        split8(ti, tih[q][hf], til[q][hf]);
+      }
+    }
+
+    half8 wrh[XW], wrl[XW], wih[XW], wil[XW];  // the current class's window
+    int wcls = -1;
+    for (int p0 = 0; p0 < sub.z; p0 += kJPiece) {
+      const int np = min(kJPiece, sub.z - p0);
+      const int64_t base = (int64_t)sub.y + p0;
+      __syncthreads();  // the previous piece's list, rows and sums are consumed
+      if (threadIdx.x < NCLS) CNT[threadIdx.x] = 0;
+      __syncthreads();
+      // class counts (one ballot per class and wave)
+      for (int pw = wave * 64; pw < np; pw += NT) {
+        const int p = pw + lane;
+        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
+#pragma unroll
+        for (int k = 0; k < NCLS; ++k) {
+          const uint64_t m = __ballot(c == k);
+          if (lane == 0 && m) atomicAdd(&CNT[k], (int)__popcll(m));
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int s = 0;
+        for (int k = 0; k < NCLS; ++k) {
+          START[k] = s;
+          CUR[k] = s;
+          s += (CNT[k] + 15) & ~15;
+        }
+        START[NCLS] = s;
+      }
+      __syncthreads();
+      // the list: each point at its class's cursor, then the pad slots
+      for (int pw = wave * 64; pw < np; pw += NT) {
+        const int p = pw + lane;
+        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
+        int pos = 0;
+#pragma unroll
+        for (int k = 0; k < NCLS; ++k) {
+          const uint64_t m = __ballot(c == k);
+          if (m) {
+            int b = 0;
+            if (lane == 0) b = atomicAdd(&CUR[k], (int)__popcll(m));
+            b = __builtin_amdgcn_readfirstlane(b);
+            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (c == k) pos = b + rank;
+          }
+        }
+        if (c >= 0) LIST[pos] = (uint16_t)p;
+      }
+      for (int t = threadIdx.x; t < 16 * NCLS; t += NT) {
+        const int k = t >> 4;
+        const int pos = START[k] + CNT[k] + (t & 15);
+        if (pos < START[k + 1]) LIST[pos] = 0xFFFF;
+      }
+      __syncthreads();
+      int cstart[NCLS + 1];
+#pragma unroll
+      for (int k = 0; k <= NCLS; ++k) cstart[k] = __builtin_amdgcn_readfirstlane(START[k]);
+      const int total = cstart[NCLS];
+
+      for (int r0 = 0; r0 < total; r0 += kJNP) {
+        const int nslot = min(kJNP, total - r0);  // a multiple of 16
+        if (r0 > 0) __syncthreads();  // the previous round's rows and sums are consumed
+        // stage: slot s holds list entry r0 + s; the y offset becomes relative
+        // to the window of the slot's class
+        for (int s = threadIdx.x; s < nslot; s += NT) {
+          const int li = LIST[r0 + s];
+          const bool ok = li != 0xFFFF;
+          const int gpos = r0 + (s & ~15);
+          int c = 0;
+#pragma unroll
+          for (int k = 1; k < NCLS; ++k) c += gpos >= cstart[k];
+#pragma unroll
+          for (int i = 0; i < 32; i += 8) {
+            *(float4 *)(BH + s * kJRowB + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+            *(float4 *)(BL + s * kJRowB + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+#pragma unroll
+          for (int i = 0; i < 16; i += 4) {
+            *(float4 *)(WY + s * kJRowY + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+            *(float4 *)(WX + s * kJRowX + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
+          const float4 z = ok ? zr.r[2 * (base + li)] : make_float4(0.f, 0.f, 0.f, 0.f);
+          ZS[s] = z.x;
+          ZS[kJNP + s] = z.y;
+          ZS[2 * kJNP + s] = z.z;
+          OFS[s] = ok ? __float_as_int(z.w) - (MW::origin(c) << 8) : 0;
+          JIDX[s] = ok ? (int)__float_as_uint(zr.r[2 * (base + li) + 1].w) : -1;
+        }
+        __syncthreads();
+        // tap weights: a task evaluates all taps of one axis for slots s, s + nslot / 2;
+        // the z weights go to LDS as the B operands (scaled, FP16 hi + lo as split8)
+        {
+          const int hn = nslot >> 1;
+          for (int u = threadIdx.x; u < 3 * hn; u += NT) {
+            const int a = (u >= hn) + (u >= 2 * hn);
+            const int s0 = u - a * hn, s1 = s0 + hn;
+            const int o0 = (OFS[s0] >> (8 * a)) & 255, o1 = (OFS[s1] >> (8 * a)) & 255;
+            float v0[2 * W], v1[2 * W];
+            taps2<W>(PL, ZS[a * kJNP + s0], ZS[a * kJNP + s1], v0, v1);
+            if (a == 2) {
+#pragma unroll
+              for (int t = 0; t < 2 * W; ++t) {
+                const float2v x = {v0[t] * kOpScale, v1[t] * kOpScale};
+                const half2v h = __builtin_convertvector(x, half2v);
+                typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
+                const fp16x2 l = __builtin_amdgcn_cvt_pkrtz(x[0] - (float)h[0], x[1] - (float)h[1]);
+                BH[s0 * kJRowB + o0 + t] = h[0];
+                BH[s1 * kJRowB + o1 + t] = h[1];
+                BL[s0 * kJRowB + o0 + t] = (_Float16)l[0];
+                BL[s1 * kJRowB + o1 + t] = (_Float16)l[1];
+              }
+            } else {
+              float *tab = a == 0 ? WX : WY;
+              const int stride = a == 0 ? kJRowX : kJRowY;
+#pragma unroll
+              for (int t = 0; t < 2 * W; ++t) {
+                tab[s0 * stride + o0 + t] = v0[t];
+                tab[s1 * stride + o1 + t] = v1[t];
+              }
+            }
+          }
+        }
+        __syncthreads();
+        for (int gi = 0; gi < (nslot >> 4); ++gi) {
+          const int gpos = r0 + 16 * gi;
+          int c = 0;
+#pragma unroll
+          for (int k = 1; k < NCLS; ++k) c += gpos >= cstart[k];
+          if (c != wcls) {
+            wcls = c;
+            const int o = MW::origin(c);
+            const int addr = 4 * (16 * g4 + ((i16 + o) & 15));
+#pragma unroll
+            for (int q = 0; q < XW; ++q) {
+              wrh[q] = window_rows(trh[q][0], trh[q][1], o, i16, addr);
+              wrl[q] = window_rows(trl[q][0], trl[q][1], o, i16, addr);
+              wih[q] = window_rows(tih[q][0], tih[q][1], o, i16, addr);
+              wil[q] = window_rows(til[q][0], til[q][1], o, i16, addr);
+            }
+          }
+          const int k = 16 * gi + i16;  // this lane's slot
+          const half8 bh = *(const half8 *)(BH + k * kJRowB + 8 * g4);
+          const half8 bl = *(const half8 *)(BL + k * kJRowB + 8 * g4);
+          // wy at this lane's D rows (window rows 4 g4 .. 4 g4 + 3), wx at the wave's cells
+          const float4 wy = *(const float4 *)(WY + k * kJRowY + 4 * g4);
+          const float4 wx4 = *(const float4 *)(WX + k * kJRowX + wave * XW);
+          const float wx[4] = {wx4.x, wx4.y, wx4.z, wx4.w};
+          float accr = 0.f, acci = 0.f;
+          // products of cell q + 1 are issued before the contraction of cell q
+          const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+          f32x4 dr[2], di[2];
+#pragma unroll
+          for (int q = 0; q <= XW; ++q) {
+            if (q < XW) {
+              f32x4 &r = dr[q & 1], &i = di[q & 1];
+              r = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrh[q], bh, zero, 0, 0, 0);
+              i = __builtin_amdgcn_mfma_f32_16x16x32_f16(wih[q], bh, zero, 0, 0, 0);
+              r = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrh[q], bl, r, 0, 0, 0);
+              i = __builtin_amdgcn_mfma_f32_16x16x32_f16(wih[q], bl, i, 0, 0, 0);
+              r = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrl[q], bh, r, 0, 0, 0);
+              i = __builtin_amdgcn_mfma_f32_16x16x32_f16(wil[q], bh, i, 0, 0, 0);
+            }
+            if (q > 0) {
+              const f32x4 &r = dr[(q - 1) & 1], &i = di[(q - 1) & 1];
+              const float pr = fmaf(wy.x, r[0], fmaf(wy.y, r[1], fmaf(wy.z, r[2], wy.w * r[3])));
+              const float pi = fmaf(wy.x, i[0], fmaf(wy.y, i[1], fmaf(wy.z, i[2], wy.w * i[3])));
+              accr = fmaf(wx[q - 1], pr, accr);
+              acci = fmaf(wx[q - 1], pi, acci);
+            }
+          }
+          // the four lane rows' partial sums are added in the output pass
+          RED[(wave * 4 + g4) * kJNP + k] = make_float2(accr, acci);
+        }
+        __syncthreads();
+        for (int s = threadIdx.x; s < nslot; s += NT) {
+          const int j = JIDX[s];
+          if (j < 0) continue;
+          float2 v = RED[s];
+#pragma unroll
+          for (int r = 1; r < 4 * NW; ++r) {
+            const float2 u = RED[r * kJNP + s];
+            v.x += u.x;
+            v.y += u.y;
+          }
+          out[j] = make_float2(v.x * s_out, v.y * s_out);
+        }
+      }
+    }
+  }
+}
+
 // ---- MFMA coordinate gradient --------------------------------------------
 //
 // With D_i[y][k] = sum_z G_i[y][z] wz_k(z) and E_i[y][k] = sum_z G_i[y][z]
@@ -1920,12 +2254,22 @@ struct Interp3Mfma {
   static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
                         float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
     const float *tmax = bmax + g.nbin[0] * g.nbin[1] * g.nbin[2];  // from TileMax3
-    constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
-    hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                       zr, grid, (const float *)tmax, out, g, dpoly);
+    if constexpr (W <= 7) {
+      // 16-row windows hold the 2w tap rows of a class of points
+      constexpr int lds = JLds::SIZE;
+      hipError_t e = hipFuncSetAttribute((const void *)interp3d_win_kernel<W>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(interp3d_win_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
+                         zr, grid, (const float *)tmax, out, g, dpoly);
+    } else {
+      constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
+      hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
+                         zr, grid, (const float *)tmax, out, g, dpoly);
+    }
     return hipGetLastError();
   }
 };
