@@ -165,27 +165,6 @@ __global__ __launch_bounds__(256) void point_prep_kernel(PointArgs pa, const flo
   }
 }
 
-// The polynomial of one tap (wave-uniform index i), coefficients read once
-// with scalar loads; value at argument z with kb_taps's endpoint rule.
-template <int W>
-struct TapPoly {
-  float c[kPolyCoefs];
-  int i;
-  // coefficient table coef[d][tap]: KbPoly::c (values) or KbPoly::dc (derivatives)
-  __device__ __forceinline__ TapPoly(const float (*__restrict__ coef)[kMaxTaps], int tap) : i(tap) {
-#pragma unroll
-    for (int d = 0; d < kPolyCoefs; ++d) c[d] = coef[d][tap];
-  }
-  __device__ __forceinline__ float operator()(float z) const {
-    float v = c[kPolyCoefs - 1];
-#pragma unroll
-    for (int d = kPolyCoefs - 2; d >= 0; --d) v = fmaf(v, z, c[d]);
-    if (i == 0 && z <= -1.0f) v = 0.0f;
-    if (i == 2 * W - 1 && z >= 1.0f) v = 0.0f;
-    return v;
-  }
-};
-
 // Tap polynomials staged in LDS as rows of kPolyRow floats per tap (the
 // kPolyCoefs coefficients, lowest degree first, zero padded), so that a lane
 // evaluates all taps of a point with broadcast reads of uniform addresses.
@@ -526,7 +505,7 @@ __device__ __forceinline__ void mfma_accumulate(f32x16 (&acc)[MSpread<W>::XW][2]
                                                 const KbPoly *__restrict__ dpoly, int4 sub,
                                                 int base, float s_in, char *sh) {
   using MS = MSpread<W>;
-  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW, T = 2 * W;
+  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, T = 2 * W;
   using L = MLds<PX>;
   float *WY = (float *)(sh + L::WY);
   float *WX = (float *)(sh + L::WX);
@@ -1569,7 +1548,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_win_kernel(
 //   Sz_k = sum_i wx_k(i) sum_y wy_k(y) E_i,
 // and grad_a = -scale_a Re(conj(u_k) S_a), u_k read from the point's record
 // (the binning stores u in the value fields).  Two x cells per wave (the
-// second B operand and the derivative rows need the registers).
+// second B operand and the derivative rows need the registers).  The
+// products run on the 16-row windows of the windowed interpolation kernel.
 template <int W>
 struct MGrad {
   static constexpr int PX = MSpread<W>::PX;
@@ -1577,43 +1557,94 @@ struct MGrad {
   static constexpr int NW = PX / XW;
   static constexpr int NT = 64 * NW;
 };
-constexpr int kGNP = 128;  // points staged per round (4 MFMA groups of 32)
-
+// ---- Windowed MFMA coordinate gradient (2w <= 14) --------------------------
+//
+// The sums of the gradient kernel above on the 16-row windows of the windowed
+// interpolation kernel: for a group of 16 points of one y-offset class and x
+// cell i of the wave, D_i = window x wz and E_i = window x wz' on
+// v_mfma_f32_16x16x32_f16 (B = the points' z weights and their derivatives,
+// staged as FP16 hi + lo rows), then Sx += wx'_i (wy . D_i), Sy += wx_i
+// (wy' . D_i) and Sz += wx_i (wy . E_i) over this lane's four D rows.  Eight
+// waves of two x cells each, one workgroup per CU; the tile maximum is taken
+// from the tile itself.
+constexpr int kGJNP = 128;  // slots staged per round (8 groups of 16)
 template <int NW>
-struct GLds {
-  static constexpr int ROWS = 0;                                   // float [6][kGNP][kIRow]
-  static constexpr int RED = ROWS + 6 * kGNP * kIRow * 4;          // float [NW][6][kGNP]
-  static constexpr int JIDX = RED + NW * 6 * kGNP * 4;             // int [2][kGNP]
-  static constexpr int ZS = JIDX + 2 * kGNP * 4;                   // float [3][kGNP]
-  static constexpr int OFS = ZS + 3 * kGNP * 4;                    // int [kGNP]
-  static constexpr int TMX = OFS + kGNP * 4;                       // float [16]
-  static constexpr int U = TMX + 16 * 4;                           // float2 [2][kGNP]
-  static constexpr int SIZE = U + 2 * kGNP * 8;
-  static_assert(SIZE <= 160 * 1024, "one gradient workgroup per CU");
+struct GJLds {
+  static constexpr int BH = 0;                             // half [kGJNP][kJRowB]: wz hi
+  static constexpr int BL = BH + kGJNP * kJRowB * 2;       // wz lo
+  static constexpr int DH = BL + kGJNP * kJRowB * 2;       // wz' hi
+  static constexpr int DL = DH + kGJNP * kJRowB * 2;       // wz' lo
+  static constexpr int WY = DL + kGJNP * kJRowB * 2;       // float [kGJNP][kJRowY]: wy
+  static constexpr int DY = WY + kGJNP * kJRowY * 4;       // wy'
+  static constexpr int WX = DY + kGJNP * kJRowY * 4;       // float [kGJNP][kJRowX]: wx
+  static constexpr int DX = WX + kGJNP * kJRowX * 4;       // wx'
+  static constexpr int RED = DX + kGJNP * kJRowX * 4;      // float [NW][2][6][kGJNP]
+  static constexpr int JIDX = RED + NW * 2 * 6 * kGJNP * 4;  // int [kGJNP]
+  static constexpr int U = JIDX + kGJNP * 4;               // float2 [kGJNP]
+  static constexpr int ZS = U + kGJNP * 8;                 // float [3][kGJNP]
+  static constexpr int OFS = ZS + 3 * kGJNP * 4;           // int [kGJNP]
+  static constexpr int TMX = OFS + kGJNP * 4;              // float [16]
+  static constexpr int CNT = TMX + 16 * 4;                 // int [8] counts, [8] cursors, [8] starts
+  static constexpr int LIST = CNT + 24 * 4;                // u16 [kJPiece + 16 * 8]
+  static constexpr int POLY = LIST + (kJPiece + 128) * 2;  // float [2][kMaxTaps][kPolyRow]
+  static constexpr int SIZE = POLY + 2 * kMaxTaps * kPolyRow * 4;
+  static_assert(SIZE <= 160 * 1024, "one windowed gradient workgroup per CU");
 };
 
+// FP16 hi + lo of s v0 and s v1 (split8's rounding) at row offsets o0 + t, o1 + t.
+template <int T>
+__device__ __forceinline__ void store_split_rows(_Float16 *hi, _Float16 *lo, int r0, int r1,
+                                                 const float (&v0)[T], const float (&v1)[T],
+                                                 float s) {
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    const float2v x = {v0[t] * s, v1[t] * s};
+    const half2v h = __builtin_convertvector(x, half2v);
+    typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
+    const fp16x2 l = __builtin_amdgcn_cvt_pkrtz(x[0] - (float)h[0], x[1] - (float)h[1]);
+    hi[r0 + t] = h[0];
+    hi[r1 + t] = h[1];
+    lo[r0 + t] = (_Float16)l[0];
+    lo[r1 + t] = (_Float16)l[1];
+  }
+}
 
 template <int W>
-__global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
+__global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
     PointArgs pa, ZRec zr, const float2 *grid, float *gx, float *gy, float *gz, Geom g,
     const KbPoly *__restrict__ dpoly) {
   using MG = MGrad<W>;
-  constexpr int NT = MG::NT, XW = MG::XW, NW = MG::NW;
-  using L = GLds<NW>;
+  using MW = MWin<W>;
+  constexpr int NT = MG::NT, XW = MG::XW, NW = MG::NW, NCLS = MW::NCLS;
+  static_assert(XW == 2 && MG::PX == 16 && NCLS <= 8, "windowed gradient shape");
+  using L = GJLds<NW>;
   extern __shared__ __align__(16) char shm[];
-  float *ROWS = (float *)(shm + L::ROWS);  // WX, WY, WZ, DWX, DWY, DWZ
+  _Float16 *BH = (_Float16 *)(shm + L::BH);
+  _Float16 *BL = (_Float16 *)(shm + L::BL);
+  _Float16 *DH = (_Float16 *)(shm + L::DH);
+  _Float16 *DL = (_Float16 *)(shm + L::DL);
+  float *WY = (float *)(shm + L::WY);
+  float *DY = (float *)(shm + L::DY);
+  float *WX = (float *)(shm + L::WX);
+  float *DX = (float *)(shm + L::DX);
   float *RED = (float *)(shm + L::RED);
   int *JIDX = (int *)(shm + L::JIDX);
+  float2 *U = (float2 *)(shm + L::U);
   float *ZS = (float *)(shm + L::ZS);
   int *OFS = (int *)(shm + L::OFS);
   float *tmx = (float *)(shm + L::TMX);
-  float2 *U = (float2 *)(shm + L::U);
-  auto row = [&](int which, int p) { return ROWS + (which * kGNP + p) * kIRow; };
+  int *CNT = (int *)(shm + L::CNT);
+  int *CUR = CNT + 8;
+  int *START = CNT + 16;
+  uint16_t *LIST = (uint16_t *)(shm + L::LIST);
+  float *PL = (float *)(shm + L::POLY);
+  float *PLD = PL + kMaxTaps * kPolyRow;
+  stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first subproblem's barriers
+  stage_poly_rows(dpoly->dc, 2 * W, PLD);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int h = lane >> 5, l32 = lane & 31;
+  const int i16 = lane & 15, g4 = lane >> 4;
   const XcdChunk xr(pa);
-  int rbuf = 0;  // buffer of the round's output indices and values
   for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
     const int4 sub = pa.subs[sidx];
     const int bin = sub.x;
@@ -1623,24 +1654,33 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
     const int bx0 = bx * g.bin[0];
     const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
 
-    float mx = 0.f;
-    const bool yok = l32 < g.pad[1];
-    const int gyy = wrap(oy + l32, g.m[1]);
+    // the tile: G[x][y = 16 hf + i16][z = 8 g4 + j] of this wave's XW x cells,
+    // scaled by the power of two of the workgroup's largest component
+    const int zc = 8 * g4;
+    float2 v[XW][2][8];
 #pragma unroll
     for (int q = 0; q < XW; ++q) {
       const int gxx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
-      const float2 *rowp = grid + ((int64_t)gxx * g.m[1] + gyy) * g.m[2];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int z = (j < 8 ? 0 : 8) + 8 * h + j;
-        if (yok && z < g.pad[2]) {
-          const float2 v = rowp[wrap(oz + z, g.m[2])];
-          mx = fmaxf(mx, fmaxf(fabsf(v.x), fabsf(v.y)));
-        }
+      for (int hf = 0; hf < 2; ++hf) {
+        const int ty = 16 * hf + i16;
+        const bool yok = ty < g.pad[1];
+        const float2 *rowp = grid + ((int64_t)gxx * g.m[1] + wrap(oy + ty, g.m[1])) * g.m[2];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          v[q][hf][j] = (yok && zc + j < g.pad[2]) ? rowp[wrap(oz + zc + j, g.m[2])]
+                                                    : make_float2(0.f, 0.f);
       }
     }
+    float mx = 0.f;
+#pragma unroll
+    for (int q = 0; q < XW; ++q)
+#pragma unroll
+      for (int hf = 0; hf < 2; ++hf)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) mx = fmaxf(mx, fmaxf(fabsf(v[q][hf][j].x), fabsf(v[q][hf][j].y)));
     mx = wave_max(mx);
-    __syncthreads();  // previous subproblem's readers of tmx / RED are done
+    __syncthreads();  // the previous subproblem's readers of tmx, rows and sums are done
     if (lane == 0) tmx[wave] = mx;
     __syncthreads();
     mx = tmx[0];
@@ -1650,179 +1690,220 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_mfma_kernel(
     if (mx > 0.f) (void)frexpf(mx, &e);
     e = max(-120, min(120, e));
     const float s_in = ldexpf(1.0f, -e) * kOpScale, s_out = ldexpf(1.0f, e);
-    half8 arh[XW][2], arl[XW][2], aih[XW][2], ail[XW][2];
+    half8 trh[XW][2], trl[XW][2], tih[XW][2], til[XW][2];
 #pragma unroll
-    for (int q = 0; q < XW; ++q) {
-      const int gxx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
-      const float2 *rowp = grid + ((int64_t)gxx * g.m[1] + gyy) * g.m[2];
+    for (int q = 0; q < XW; ++q)
 #pragma unroll
-      for (int st = 0; st < 2; ++st) {
+      for (int hf = 0; hf < 2; ++hf) {
         float tr[8], ti[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const int z = 16 * st + 8 * h + j;
-          float2 v = make_float2(0.f, 0.f);
-          if (yok && z < g.pad[2]) v = rowp[wrap(oz + z, g.m[2])];
-          tr[j] = v.x * s_in;
-          ti[j] = v.y * s_in;
+          tr[j] = v[q][hf][j].x * s_in;
+          ti[j] = v[q][hf][j].y * s_in;
         }
-        split8(tr, arh[q][st], arl[q][st]);
-        split8(ti, aih[q][st], ail[q][st]);
+        split8(tr, trh[q][hf], trl[q][hf]);
+        split8(ti, tih[q][hf], til[q][hf]);
       }
-    }
 
-    // records of the next round of this subproblem, loaded during the
-    // current round (thread p < kGNP holds point p)
-    float4 nza = make_float4(0.f, 0.f, 0.f, 0.f), nzb = nza;
-    bool have = false;
-    for (int c0 = 0; c0 < sub.z; c0 += kGNP) {
-      const int npts = min(kGNP, sub.z - c0);
-      const int ng = (npts + 31) >> 5;
-      const int np32 = ng << 5;
-      const int start = sub.y + c0;
-      // No barrier here: the staged rows, ZS and OFS were last read before
-      // the previous round's reduction barrier, and the output indices and
-      // values alternate between two buffers, so this round's staging may
-      // run while other waves still write the previous round's outputs
-      // (RED is rewritten only after the next two barriers).
-      int *JX = JIDX + rbuf * kGNP;
-      float2 *UX = U + rbuf * kGNP;
-      rbuf ^= 1;
-      // pass 0: zero the six staged rows (all threads); arguments, offsets,
-      // output indices and values of the points
-      for (int t = threadIdx.x; t < np32 * 48; t += NT) {
-        const int p = t / 48, r = (t >> 3) % 6, i = (t & 7) << 2;
-        *(float4 *)(row(r, p) + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-      if (threadIdx.x < np32) {
-        const int p = threadIdx.x;
-        const bool ok = p < npts;
-        float4 z = make_float4(0.f, 0.f, 0.f, 0.f), c = z;
-        if (ok) {
-          z = have ? nza : zr.r[2 * (int64_t)(start + p)];
-          c = have ? nzb : zr.r[2 * (int64_t)(start + p) + 1];
-        }
-        ZS[p] = z.x;
-        ZS[kGNP + p] = z.y;
-        ZS[2 * kGNP + p] = z.z;
-        OFS[p] = __float_as_int(z.w);
-        JX[p] = (int)__float_as_uint(c.w);
-        UX[p] = make_float2(c.x, c.y);
-      }
-      have = c0 + kGNP < sub.z;
-      if (have && threadIdx.x < min(kGNP, sub.z - c0 - kGNP)) {
-        nza = zr.r[2 * (int64_t)(start + kGNP + threadIdx.x)];
-        nzb = zr.r[2 * (int64_t)(start + kGNP + threadIdx.x) + 1];
-      }
+    half8 wrh[XW], wrl[XW], wih[XW], wil[XW];  // the current class's window
+    int wcls = -1;
+    for (int p0 = 0; p0 < sub.z; p0 += kJPiece) {
+      const int np = min(kJPiece, sub.z - p0);
+      const int64_t base = (int64_t)sub.y + p0;
+      if (p0 > 0) __syncthreads();  // the previous piece's list, rows and sums are consumed
+      if (threadIdx.x < NCLS) CNT[threadIdx.x] = 0;
       __syncthreads();
-      // pass 1: value and derivative taps, whole taps per wave
-      for (int i0 = wave; i0 < 2 * W; i0 += NW) {
-        const int i = __builtin_amdgcn_readfirstlane(i0);
-        const TapPoly<W> tv(dpoly->c, i), td(dpoly->dc, i);
-        for (int p = lane; p < np32; p += 64) {
-          const int off = OFS[p];
+      for (int pw = wave * 64; pw < np; pw += NT) {
+        const int p = pw + lane;
+        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float z = ZS[a * kGNP + p];
-            const int l = ((off >> (8 * a)) & 255) + i;
-            row(a, p)[l] = tv(z);
-            row(3 + a, p)[l] = td(z);
-          }
+        for (int k = 0; k < NCLS; ++k) {
+          const uint64_t m = __ballot(c == k);
+          if (lane == 0 && m) atomicAdd(&CNT[k], (int)__popcll(m));
         }
       }
       __syncthreads();
-      for (int gi = 0; gi < ng; ++gi) {
-        const int k = (gi << 5) + l32;
-        float wy[16];
+      if (threadIdx.x == 0) {
+        int s = 0;
+        for (int k = 0; k < NCLS; ++k) {
+          START[k] = s;
+          CUR[k] = s;
+          s += (CNT[k] + 15) & ~15;
+        }
+        START[NCLS] = s;
+      }
+      __syncthreads();
+      for (int pw = wave * 64; pw < np; pw += NT) {
+        const int p = pw + lane;
+        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
+        int pos = 0;
 #pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4)
-          *(float4 *)(wy + 4 * r4) = *(const float4 *)(row(1, k) + 8 * r4 + 4 * h);
-        float sxr = 0.f, sxi = 0.f, syr = 0.f, syi = 0.f, szr = 0.f, szi = 0.f;
-        // phase 0: B = wz (D, for Sx and Sy); phase 1: B = wz' (E, for Sz)
-#pragma unroll 1
-        for (int ph = 0; ph < 2; ++ph) {
-          half8 bh[2], bl[2];
-#pragma unroll
-          for (int st = 0; st < 2; ++st) {
-            float wz[8];
-            const float *src = row(ph == 0 ? 2 : 5, k) + 16 * st + 8 * h;
-            *(float4 *)wz = *(const float4 *)src;
-            *(float4 *)(wz + 4) = *(const float4 *)(src + 4);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) wz[j] *= ph == 0 ? kOpScale : kOpScaleD;
-            split8(wz, bh[st], bl[st]);
+        for (int k = 0; k < NCLS; ++k) {
+          const uint64_t m = __ballot(c == k);
+          if (m) {
+            int b = 0;
+            if (lane == 0) b = atomicAdd(&CUR[k], (int)__popcll(m));
+            b = __builtin_amdgcn_readfirstlane(b);
+            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (c == k) pos = b + rank;
           }
-          float dy[16];
-          if (ph == 0) {
+        }
+        if (c >= 0) LIST[pos] = (uint16_t)p;
+      }
+      for (int t = threadIdx.x; t < 16 * NCLS; t += NT) {
+        const int k = t >> 4;
+        const int pos = START[k] + CNT[k] + (t & 15);
+        if (pos < START[k + 1]) LIST[pos] = 0xFFFF;
+      }
+      __syncthreads();
+      int cstart[NCLS + 1];
 #pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4)
-              *(float4 *)(dy + 4 * r4) = *(const float4 *)(row(4, k) + 8 * r4 + 4 * h);
+      for (int k = 0; k <= NCLS; ++k) cstart[k] = __builtin_amdgcn_readfirstlane(START[k]);
+      const int total = cstart[NCLS];
+
+      for (int r0 = 0; r0 < total; r0 += kGJNP) {
+        const int nslot = min(kGJNP, total - r0);  // a multiple of 16
+        if (r0 > 0) __syncthreads();  // the previous round's rows and sums are consumed
+        // zero the eight staged rows of every slot (64 bytes each)
+        for (int t = threadIdx.x; t < nslot * 8; t += NT) {
+          const int s = t >> 3, r = t & 7;
+          float4 *dst = r < 4 ? (float4 *)((r == 0 ? BH : r == 1 ? BL : r == 2 ? DH : DL) + s * kJRowB)
+                              : (float4 *)((r == 4 ? WY : r == 5 ? DY : r == 6 ? WX : DX) +
+                                           s * (r < 6 ? kJRowY : kJRowX));
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dst[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        for (int s = threadIdx.x; s < nslot; s += NT) {
+          const int li = LIST[r0 + s];
+          const bool ok = li != 0xFFFF;
+          const int gpos = r0 + (s & ~15);
+          int c = 0;
+#pragma unroll
+          for (int k = 1; k < NCLS; ++k) c += gpos >= cstart[k];
+          const float4 z = ok ? zr.r[2 * (base + li)] : make_float4(0.f, 0.f, 0.f, 0.f);
+          const float4 rb = ok ? zr.r[2 * (base + li) + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
+          ZS[s] = z.x;
+          ZS[kGJNP + s] = z.y;
+          ZS[2 * kGJNP + s] = z.z;
+          OFS[s] = ok ? __float_as_int(z.w) - (MW::origin(c) << 8) : 0;
+          JIDX[s] = ok ? (int)__float_as_uint(rb.w) : -1;
+          U[s] = make_float2(rb.x, rb.y);  // u travels in the records' value fields
+        }
+        __syncthreads();
+        // taps: a task evaluates the value (d = 0) or derivative (d = 1) taps of
+        // one axis for slots s, s + nslot / 2
+        {
+          const int hn = nslot >> 1;
+          for (int u = threadIdx.x; u < 6 * hn; u += NT) {
+            const int ad = u / hn, a = ad >> 1, d = ad & 1;
+            const int s0 = u - ad * hn, s1 = s0 + hn;
+            const int o0 = (OFS[s0] >> (8 * a)) & 255, o1 = (OFS[s1] >> (8 * a)) & 255;
+            float v0[2 * W], v1[2 * W];
+            taps2<W>(d ? PLD : PL, ZS[a * kGJNP + s0], ZS[a * kGJNP + s1], v0, v1);
+            if (a == 2) {
+              store_split_rows<2 * W>(d ? DH : BH, d ? DL : BL, s0 * kJRowB + o0, s1 * kJRowB + o1,
+                                      v0, v1, d ? kOpScaleD : kOpScale);
+            } else {
+              float *tab = a == 0 ? (d ? DX : WX) : (d ? DY : WY);
+              const int stride = a == 0 ? kJRowX : kJRowY;
+#pragma unroll
+              for (int t = 0; t < 2 * W; ++t) {
+                tab[s0 * stride + o0 + t] = v0[t];
+                tab[s1 * stride + o1 + t] = v1[t];
+              }
+            }
           }
+        }
+        __syncthreads();
+        for (int gi = 0; gi < (nslot >> 4); ++gi) {
+          const int gpos = r0 + 16 * gi;
+          int c = 0;
+#pragma unroll
+          for (int k = 1; k < NCLS; ++k) c += gpos >= cstart[k];
+          if (c != wcls) {
+            wcls = c;
+            const int o = MW::origin(c);
+            const int addr = 4 * (16 * g4 + ((i16 + o) & 15));
+#pragma unroll
+            for (int q = 0; q < XW; ++q) {
+              wrh[q] = window_rows(trh[q][0], trh[q][1], o, i16, addr);
+              wrl[q] = window_rows(trl[q][0], trl[q][1], o, i16, addr);
+              wih[q] = window_rows(tih[q][0], tih[q][1], o, i16, addr);
+              wil[q] = window_rows(til[q][0], til[q][1], o, i16, addr);
+            }
+          }
+          const int k = 16 * gi + i16;  // this lane's slot
+          const half8 bh = *(const half8 *)(BH + k * kJRowB + 8 * g4);
+          const half8 bl = *(const half8 *)(BL + k * kJRowB + 8 * g4);
+          const half8 dh = *(const half8 *)(DH + k * kJRowB + 8 * g4);
+          const half8 dl = *(const half8 *)(DL + k * kJRowB + 8 * g4);
+          // wy, wy' at this lane's D rows (window rows 4 g4 .. 4 g4 + 3); wx, wx'
+          // at the wave's cells
+          const float4 wy = *(const float4 *)(WY + k * kJRowY + 4 * g4);
+          const float4 dy = *(const float4 *)(DY + k * kJRowY + 4 * g4);
+          const float2 wx = *(const float2 *)(WX + k * kJRowX + wave * XW);
+          const float2 dx = *(const float2 *)(DX + k * kJRowX + wave * XW);
+          const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+          f32x4 dr[XW], di[XW], er[XW], ei[XW];
 #pragma unroll
           for (int q = 0; q < XW; ++q) {
-            const float wx = row(0, k)[wave * XW + q];
-            f32x16 dr, di;
+            dr[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrh[q], bh, zero, 0, 0, 0);
+            di[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wih[q], bh, zero, 0, 0, 0);
+            er[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrh[q], dh, zero, 0, 0, 0);
+            ei[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wih[q], dh, zero, 0, 0, 0);
+            dr[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrh[q], bl, dr[q], 0, 0, 0);
+            di[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wih[q], bl, di[q], 0, 0, 0);
+            er[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrh[q], dl, er[q], 0, 0, 0);
+            ei[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wih[q], dl, ei[q], 0, 0, 0);
+            dr[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrl[q], bh, dr[q], 0, 0, 0);
+            di[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wil[q], bh, di[q], 0, 0, 0);
+            er[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wrl[q], dh, er[q], 0, 0, 0);
+            ei[q] = __builtin_amdgcn_mfma_f32_16x16x32_f16(wil[q], dh, ei[q], 0, 0, 0);
+          }
+          float sv[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // Sx, Sy, Sz (re, im)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              dr[r] = 0.f;
-              di[r] = 0.f;
-            }
+          for (int q = 0; q < XW; ++q) {
+            const float xw = q == 0 ? wx.x : wx.y, xd = q == 0 ? dx.x : dx.y;
+            const f32x4 &r = dr[q], &i = di[q];
+            const float ar = fmaf(wy.x, r[0], fmaf(wy.y, r[1], fmaf(wy.z, r[2], wy.w * r[3])));
+            const float ai = fmaf(wy.x, i[0], fmaf(wy.y, i[1], fmaf(wy.z, i[2], wy.w * i[3])));
+            const float br = fmaf(dy.x, r[0], fmaf(dy.y, r[1], fmaf(dy.z, r[2], dy.w * r[3])));
+            const float bi = fmaf(dy.x, i[0], fmaf(dy.y, i[1], fmaf(dy.z, i[2], dy.w * i[3])));
+            const float cr = fmaf(wy.x, er[q][0], fmaf(wy.y, er[q][1], fmaf(wy.z, er[q][2], wy.w * er[q][3])));
+            const float ci = fmaf(wy.x, ei[q][0], fmaf(wy.y, ei[q][1], fmaf(wy.z, ei[q][2], wy.w * ei[q][3])));
+            sv[0] = fmaf(xd, ar, sv[0]);
+            sv[1] = fmaf(xd, ai, sv[1]);
+            sv[2] = fmaf(xw, br, sv[2]);
+            sv[3] = fmaf(xw, bi, sv[3]);
+            sv[4] = fmaf(xw, cr, sv[4]);
+            sv[5] = fmaf(xw, ci, sv[5]);
+          }
+          // lane rows g4 and g4 + 2 are added here, rows 0 / 1 in the output pass
 #pragma unroll
-            for (int st = 0; st < 2; ++st) {
-              dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bh[st], dr, 0, 0, 0);
-              dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bl[st], dr, 0, 0, 0);
-              dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl[q][st], bh[st], dr, 0, 0, 0);
-              di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bh[st], di, 0, 0, 0);
-              di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bl[st], di, 0, 0, 0);
-              di = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail[q][st], bh[st], di, 0, 0, 0);
-            }
-            float ar = 0.f, ai = 0.f;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              ar = fmaf(wy[r], dr[r], ar);
-              ai = fmaf(wy[r], di[r], ai);
-            }
-            if (ph == 0) {
-              const float dwx = row(3, k)[wave * XW + q];
-              float br = 0.f, bi = 0.f;
-#pragma unroll
-              for (int r = 0; r < 16; ++r) {
-                br = fmaf(dy[r], dr[r], br);
-                bi = fmaf(dy[r], di[r], bi);
-              }
-              sxr = fmaf(dwx, ar, sxr);
-              sxi = fmaf(dwx, ai, sxi);
-              syr = fmaf(wx, br, syr);
-              syi = fmaf(wx, bi, syi);
-            } else {
-              szr = fmaf(wx, ar, szr);
-              szi = fmaf(wx, ai, szi);
-            }
+          for (int c6 = 0; c6 < 6; ++c6) {
+            sv[c6] += __shfl_xor(sv[c6], 32);
+            if (g4 < 2) RED[((wave * 2 + g4) * 6 + c6) * kGJNP + k] = sv[c6];
           }
         }
-        float sv[6] = {sxr, sxi, syr, syi, szr, szi};
+        __syncthreads();
+        for (int s = threadIdx.x; s < nslot; s += NT) {
+          const int j = JIDX[s];
+          if (j < 0) continue;
+          float sv[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          sv[c] += __shfl_xor(sv[c], 32);
-          if (h == 0) RED[(wave * 6 + c) * kGNP + k] = sv[c];
+          for (int c6 = 0; c6 < 6; ++c6) {
+            float a = 0.f;
+#pragma unroll
+            for (int r = 0; r < 2 * NW; ++r) a += RED[(r * 6 + c6) * kGJNP + s];
+            sv[c6] = a * s_out * (c6 < 4 ? kProdUnscale : kProdUnscaleD);
+          }
+          const float2 uj = U[s];
+          // Re(conj(u) * (-s) * S) = -s (u.x S.x + u.y S.y)
+          gx[j] = -(float)g.scale[0] * (uj.x * sv[0] + uj.y * sv[1]);
+          if (gy) gy[j] = -(float)g.scale[1] * (uj.x * sv[2] + uj.y * sv[3]);
+          if (gz) gz[j] = -(float)g.scale[2] * (uj.x * sv[4] + uj.y * sv[5]);
         }
-      }
-      __syncthreads();
-      for (int p = threadIdx.x; p < npts; p += NT) {
-        float sv[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-          float a = 0.f;
-#pragma unroll
-          for (int w2 = 0; w2 < NW; ++w2) a += RED[(w2 * 6 + c) * kGNP + p];
-          sv[c] = a * s_out * (c < 4 ? kProdUnscale : kProdUnscaleD);
-        }
-        const int j = JX[p];
-        const float2 uj = UX[p];
-        // Re(conj(u) * (-s) * S) = -s (u.x S.x + u.y S.y)
-        gx[j] = -(float)g.scale[0] * (uj.x * sv[0] + uj.y * sv[1]);
-        if (gy) gy[j] = -(float)g.scale[1] * (uj.x * sv[2] + uj.y * sv[3]);
-        if (gz) gz[j] = -(float)g.scale[2] * (uj.x * sv[4] + uj.y * sv[5]);
       }
     }
   }
@@ -2235,7 +2316,6 @@ struct TileMax3 {
     // them) -> tile maxima (second half)
     const int nbins = g.nbin[0] * g.nbin[1] * g.nbin[2];
     float *tmax = bmax + nbins;
-    const unsigned tb = (unsigned)((nbins + 63) / 64);
     // tile_max_kernel lists at most kTileMaxList bins per axis
     if (MSpread<W>::PX / g.bin[0] + 2 > kTileMaxList || g.pad[1] / g.bin[1] + 2 > kTileMaxList ||
         g.pad[2] / g.bin[2] + 2 > kTileMaxList)
@@ -2294,13 +2374,13 @@ struct Grad3Mfma {
                         float *gx, float *gy, float *gz, const Geom &g, const KbPoly *dpoly,
                         int nblk, hipStream_t s) {
     if constexpr (W < 8) {
-      constexpr int lds = GLds<MGrad<W>::NW>::SIZE;
-      hipError_t e = hipFuncSetAttribute((const void *)grad3d_mfma_kernel<W>,
+      constexpr int lds = GJLds<MGrad<W>::NW>::SIZE;
+      hipError_t e = hipFuncSetAttribute((const void *)grad3d_win_kernel<W>,
                                          hipFuncAttributeMaxDynamicSharedMemorySize, lds);
       if (e != hipSuccess) return e;
       // u reaches the kernel through the records' value fields
       (void)u;
-      hipLaunchKernelGGL(grad3d_mfma_kernel<W>, dim3(nblk), dim3(MGrad<W>::NT), lds, s, pa, zr,
+      hipLaunchKernelGGL(grad3d_win_kernel<W>, dim3(nblk), dim3(MGrad<W>::NT), lds, s, pa, zr,
                          grid, gx, gy, gz, g, dpoly);
       return hipGetLastError();
     } else {
