@@ -432,6 +432,44 @@ def test_grad_points_type2_vs_direct(dev, oracle_mod, method, w):
     assert rel_l2(out.cpu().numpy(), ref) < TOL
 
 
+@pytest.mark.parametrize("w", [4, 6, 7])
+def test_windowed_kernels_large_subproblems(dev, oracle_mod, w):
+    """Type-2 and the coordinate gradient with one dense tile of ~40,000
+    points and max_subproblem = 16384: the windowed kernels list a
+    subproblem's points by y-offset class in pieces of 4096, so this runs
+    several pieces per subproblem and several subproblems per tile.  A
+    point's sum does not depend on which points share its group or piece,
+    so the results equal those of 64-point subproblems bit for bit; both
+    are checked against FP64 direct sums on sampled points."""
+    import torch
+    rng = np.random.default_rng(91)
+    n = (24, 20, 28)
+    big, d = _plan3(n, w=w, method="mfma", max_sub=16384)
+    small, _ = _plan3(n, w=w, method="mfma", max_sub=64)
+    M = 60000
+    xyz = uniform_points(rng, M)
+    mc = int(0.7 * M)
+    xyz[:mc] = (0.4 + 0.01 * rng.standard_normal((mc, 3))).astype(np.float32)
+    f = cnormal(rng, n)
+    u = cnormal(rng, M)
+    tx, tf, tu = _t(xyz, dev), _t(f, dev), _t(u, dev)
+    print(f"windowed w={w}: M {M}", flush=True)
+    o_big = big.type2(tx, tf).cpu().numpy()
+    o_small = small.type2(tx, tf).cpu().numpy()
+    g_big = big.grad_points(tx, tu, tf).cpu().numpy()
+    g_small = small.grad_points(tx, tu, tf).cpu().numpy()
+    torch.cuda.synchronize()
+    assert np.array_equal(o_big, o_small)
+    assert np.array_equal(g_big, g_small)
+    x64 = xyz.astype(np.float64)
+    pj = np.concatenate([rng.choice(mc, 600, replace=False), mc + rng.choice(M - mc, 200, replace=False)])
+    ref = oracle_mod.direct_type2_3d(f.astype(np.complex128), x64[pj], d)
+    assert rel_l2(o_big[pj], ref) < TOL
+    ref_x = oracle_mod.direct_type2_3d_grad_x(f.astype(np.complex128), x64[pj], d,
+                                              u[pj].astype(np.complex128) / 2.0)
+    assert rel_l2(g_big[pj], ref_x) < TOL
+
+
 @pytest.mark.parametrize("w", [4, 7])
 def test_type2_backward_matches_separate_calls(dev, oracle_mod, w):
     """The fused Type-2 backward (one binning) against the Type-1 of g and the
