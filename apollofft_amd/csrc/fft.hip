@@ -32,6 +32,13 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // multiply by -i
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
+// x / d for x d < 2^32, with m = ceil(2^32 / d) (d > 1): floor(x m / 2^32)
+// exceeds x / d by less than x / 2^32 < 1 / d, which cannot reach the next
+// integer.  The index arithmetic of a pass stays below 2^13 x 2^13.
+__device__ __forceinline__ int fdiv(int x, int d, uint32_t m) {
+  return d == 1 ? x : (int)__umulhi((uint32_t)x, m);
+}
+
 template <int R>
 __device__ __forceinline__ void small_dft(float2 *v, const float2 *tw, int L);
 
@@ -104,13 +111,13 @@ __device__ __forceinline__ void small_dft<5>(float2 *v, const float2 *tw, int L)
 //   dst[(j - k) R + k + u Ns] = DFT_R(v)_u.
 template <int R>
 __device__ __forceinline__ void stage(const float2 *src, float2 *dst, int L, int Ns,
-                                      const float2 *tw, int lines) {
+                                      const float2 *tw, int lines, uint32_t mag_nbf,
+                                      uint32_t mag_ns, int twstep) {
   const int nbf = L / R;
-  const int twstep = L / (Ns * R);
   for (int it = threadIdx.x; it < lines * nbf; it += kFftThreads) {
-    int li = it / nbf;
-    int j = it - li * nbf;
-    int k = j % Ns;
+    const int li = fdiv(it, nbf, mag_nbf);
+    const int j = it - li * nbf;
+    const int k = j - fdiv(j, Ns, mag_ns) * Ns;
     const float2 *s = src + li * L;
     float2 v[R];
 #pragma unroll
@@ -166,12 +173,14 @@ __device__ __forceinline__ void run_stages(const FftPass &p, int Lf, const float
   for (int f = 0; f < p.nfac; ++f) {
     __syncthreads();
     const int R = p.fac[f];
+    const uint32_t mb = p.mag_nbf[f], ms = p.mag_ns[f];
+    const int ts = p.twstep[f];
     switch (R) {
-    case 2: stage<2>(src, dst, Lf, Ns, tw, nl); break;
-    case 3: stage<3>(src, dst, Lf, Ns, tw, nl); break;
-    case 4: stage<4>(src, dst, Lf, Ns, tw, nl); break;
-    case 5: stage<5>(src, dst, Lf, Ns, tw, nl); break;
-    case 8: stage<8>(src, dst, Lf, Ns, tw, nl); break;
+    case 2: stage<2>(src, dst, Lf, Ns, tw, nl, mb, ms, ts); break;
+    case 3: stage<3>(src, dst, Lf, Ns, tw, nl, mb, ms, ts); break;
+    case 4: stage<4>(src, dst, Lf, Ns, tw, nl, mb, ms, ts); break;
+    case 5: stage<5>(src, dst, Lf, Ns, tw, nl, mb, ms, ts); break;
+    case 8: stage<8>(src, dst, Lf, Ns, tw, nl, mb, ms, ts); break;
     default: stage_generic(src, dst, Lf, Ns, R, tw, nl); break;
     }
     float2 *t = src;
@@ -196,6 +205,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
       p.zero[i] = 0.f;
   const int64_t l0 = (int64_t)blockIdx.x * lb;
   const int nl = (int)((p.nlines - l0) < lb ? (p.nlines - l0) : lb);
+  const uint32_t mag_nl = nl == lb ? p.mag_lb : p.mag_last;  // nl < lb only in the last block
   // element offsets of the block's lines, one 64-bit division per line
   for (int li = threadIdx.x; li < nl; li += kFftThreads) {
     const int64_t l = l0 + li, a = l / p.nb, b = l - a * p.nb;
@@ -214,10 +224,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   for (int idx = threadIdx.x; idx < nl * p.nin; idx += kFftThreads) {
     int li, i;
     if (in_contig) {
-      li = idx / p.nin;
+      li = fdiv(idx, p.nin, p.mag_nin);
       i = idx - li * p.nin;
     } else {
-      i = idx / nl;
+      i = fdiv(idx, nl, mag_nl);
       li = idx - i * nl;
     }
     float2 v = p.in[lbase[li] + (int64_t)i * p.in_si];
@@ -246,7 +256,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     // C = FFT(a) FFT(b) / Lf; the inverse FFT of C is conj(FFT(conj(C)))
     __syncthreads();
     for (int idx = threadIdx.x; idx < nl * Lf; idx += kFftThreads) {
-      const float2 c = cmul(src[idx], p.bhat[idx % Lf]);
+      const float2 c = cmul(src[idx], p.bhat[idx & (Lf - 1)]);  // Lf: a power of two
       src[idx] = make_float2(c.x, -c.y);
     }
     run_stages(p, Lf, tw, nl, src, dst);
@@ -267,10 +277,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   for (int idx = threadIdx.x; idx < nl * p.nout; idx += kFftThreads) {
     int li, o;
     if (out_contig) {
-      li = idx / p.nout;
+      li = fdiv(idx, p.nout, p.mag_nout);
       o = idx - li * p.nout;
     } else {
-      o = idx / nl;
+      o = fdiv(idx, nl, mag_nl);
       li = idx - o * nl;
     }
     const int pos = kept_pos(o, p.nout, L);
@@ -281,7 +291,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     v.x *= f;
     v.y *= f;
     p.out[lbase[lb + li] + (int64_t)o * p.out_si] = v;
-    if (p.bmax) atomicMax(&bm[li * nbz + o / p.bm_bin[2]], __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
+    if (p.bmax)
+      atomicMax(&bm[li * nbz + fdiv(o, p.bm_bin[2], p.mag_bmz)], __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
   }
   if (p.bmax) {
     __syncthreads();
@@ -309,8 +320,29 @@ __global__ void fill_kernel(float2 *p, int64_t n) {
 
 } // namespace
 
-hipError_t launch_fft_pass(const FftPass &p, hipStream_t s) {
-  if (p.nlines <= 0) return hipSuccess;
+// ceil(2^32 / d) for d > 1, 0 for d <= 1 (fdiv's multiplier)
+static uint32_t div_magic(int64_t d) {
+  return d > 1 ? (uint32_t)(((uint64_t)1 << 32) / (uint64_t)d + ((((uint64_t)1 << 32) % (uint64_t)d) ? 1 : 0)) : 0u;
+}
+
+hipError_t launch_fft_pass(const FftPass &pass, hipStream_t s) {
+  if (pass.nlines <= 0) return hipSuccess;
+  FftPass p = pass;
+  p.mag_nin = div_magic(p.nin);
+  p.mag_nout = div_magic(p.nout);
+  p.mag_bmz = div_magic(p.bmax ? p.bm_bin[2] : 1);
+  p.mag_lb = div_magic(p.lb);
+  p.mag_last = div_magic(p.nlines - (p.nlines - 1) / p.lb * p.lb);
+  {
+    const int Lf0 = p.blue_n > 0 ? p.blue_n : p.L;
+    int64_t ns = 1;
+    for (int f = 0; f < p.nfac && f < 16; ++f) {
+      p.mag_nbf[f] = div_magic(Lf0 / p.fac[f]);
+      p.mag_ns[f] = div_magic(ns);
+      p.twstep[f] = (int)(Lf0 / (ns * p.fac[f]));
+      ns *= p.fac[f];
+    }
+  }
   int64_t blocks = (p.nlines + p.lb - 1) / p.lb;
   const int Lf = p.blue_n > 0 ? p.blue_n : p.L;
   size_t lds = (2 * (size_t)p.lb + 1) * Lf * sizeof(float2) + 2 * (size_t)p.lb * sizeof(int64_t);

@@ -121,6 +121,15 @@ struct FftPass {
   int bm_nbin[3];
   float *zero;
   int zero_n;
+  // Division by the pass's runtime divisors as multiply-high by
+  // ceil(2^32 / d) (0 for d = 1), set by launch_fft_pass: nin, nout, the
+  // z bin width of the bin maxima, the lines of a full and of the last
+  // block, and per stage f the butterflies per line Lf / fac[f] and the
+  // stage span Ns (product of fac[0 .. f-1]); the twiddle step of stage f,
+  // Lf / (Ns fac[f]).
+  uint32_t mag_nin, mag_nout, mag_bmz, mag_lb, mag_last;
+  uint32_t mag_nbf[16], mag_ns[16];
+  int twstep[16];
 };
 
 constexpr int kDirectPrimeMax = 16;   // larger prime radices go through Bluestein
