@@ -109,6 +109,7 @@ CASES_3D = [
     ((8, 8, 8), 2, "reference"),
     ((20, 24, 18), 5, "exact"),
     ((127, 6, 10), 6, "exact"),   # m = 254 = 2 * 127: Bluestein, 512-point FFTs
+    ((10, 12, 14), 3, "reference"),
 ]
 
 
@@ -410,10 +411,11 @@ def test_1d_type1_type2_vs_direct(dev, oracle_mod, n, w):
 
 # ---------------------------------------------------------------- gradient
 @pytest.mark.parametrize("method", METHODS)
-@pytest.mark.parametrize("w", [4, 7, 8])
+@pytest.mark.parametrize("w", [4, 5, 6, 7, 8])
 def test_grad_points_type2_vs_direct(dev, oracle_mod, method, w):
     """d/dx of L = ||T2(f,x) - b||^2 against the closed-form direct sum
-    (w = 8 runs the LDS-tile gradient kernel under both methods)."""
+    (the windowed MFMA kernel with 3 to 6 y-offset classes; w = 8 runs the
+    LDS-tile gradient kernel under both methods)."""
     import torch
     rng = np.random.default_rng(3)
     n = (12, 12, 12)
@@ -430,6 +432,50 @@ def test_grad_points_type2_vs_direct(dev, oracle_mod, method, w):
     out = plan.grad_points(_t(xyz, dev), _t(g, dev), _t(f, dev))
     torch.cuda.synchronize()
     assert rel_l2(out.cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("w", [2, 3])
+def test_grad_points_narrow_kernels(dev, oracle_mod, w):
+    """At w = 2 and 3 the kernel's own truncation error exceeds 1e-5, so the
+    gradient is pinned to the derivative of the same approximation: FP64
+    central differences (h = 1e-7) of the oracle's fast Type-2 restatement,
+    at points more than 1e-4 cells from a cell boundary on every axis (the
+    tap set changes only at cell boundaries, and inside a cell the tap
+    weights are smooth in the coordinate).  The windowed MFMA kernel (3
+    classes of 13 / 11 offsets) and the LDS-tile vector-ALU kernel must both
+    match it, and each other, to 1e-5."""
+    import torch
+    rng = np.random.default_rng(5 + w)
+    n = (12, 12, 12)
+    pm, d = _plan3(n, w=w, method="mfma")
+    pv, _ = _plan3(n, w=w, method="valu")
+    m = pm.meta["m"]
+    M = 1500
+    xyz = uniform_points(rng, M)
+    f = cnormal(rng, n)
+    u = cnormal(rng, M)
+    tx, tf, tu = _t(xyz, dev), _t(f, dev), _t(u, dev)
+    gm = pm.grad_points(tx, tu, tf).cpu().numpy()
+    gv = pv.grad_points(tx, tu, tf).cpu().numpy()
+    torch.cuda.synchronize()
+    assert rel_l2(gm, gv) < TOL
+    x64 = xyz.astype(np.float64)
+    f64 = f.astype(np.complex128)
+    t = x64 * (np.array(m, dtype=np.float64) / (np.array(n) * np.array(d)))
+    fr = t - np.floor(t)
+    keep = np.all(np.minimum(fr, 1.0 - fr) > 1e-4, axis=1)
+    assert keep.sum() > 0.99 * M
+    h = 1e-7
+    fd = np.empty((M, 3))
+    for a in range(3):
+        xp, xm = x64.copy(), x64.copy()
+        xp[:, a] += h
+        xm[:, a] -= h
+        dt = (oracle_mod.nufft3d_type2_fast(f64, xp, d, 2, w, 0) -
+              oracle_mod.nufft3d_type2_fast(f64, xm, d, 2, w, 0)) / (2 * h)
+        fd[:, a] = np.real(np.conj(u.astype(np.complex128)) * dt)
+    assert rel_l2(gm[keep], fd[keep]) < TOL
+    assert rel_l2(gv[keep], fd[keep]) < TOL
 
 
 @pytest.mark.parametrize("w", [4, 6, 7])
