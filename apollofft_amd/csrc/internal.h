@@ -276,6 +276,7 @@ inline int bin1d(int w) { return kTile1d - 2 * w + 1; }
 // LDS bytes of the 3D spread/interp kernels for a geometry.
 size_t spread_lds_bytes(int w, int method, const Geom &g);
 int spread_bin_x(int w, int method);        // spreading tile interior along x
+int interp_blocks_per_cu(int w, int method);  // workgroups per CU of 3D interpolation
 // The MFMA gradient kernel keeps two x cells per wave; at w = 8 (32-cell
 // tiles, 16 waves) it would not fit the registers, so w = 8 uses LDS tiles.
 inline bool grad_uses_mfma(int w, int method) { return method == kSpreadMfma && w < 8; }

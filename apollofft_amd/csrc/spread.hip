@@ -994,228 +994,18 @@ __global__ __launch_bounds__(64) void tile_max_kernel(const float *bmax, Geom g,
   if (threadIdx.x == 0) tmax[bin] = mx;
 }
 
-// ---- MFMA interpolation -----------------------------------------------------
+// ---- MFMA interpolation on 16-row windows ----------------------------------
 //
-// Same tile geometry as MFMA spreading.  For x cell i the wave computes
-// D_i[y][k] = sum_z G_i[y][z] wz_k(z) (A = the tile plane, 32 y x 32 z, held
-// in registers for the whole subproblem; B = the z weights of 32 points),
-// then out_k += wx_k(i) sum_y wy_k(y) D_i[y][k] on the vector ALUs.  The
-// tile is scaled by a power of two so that its largest component is in
-// [1/2, 1) and split into FP16 hi + lo like the spreading operands.
-constexpr int kINP = 128;       // points staged per round (4 MFMA groups of 32)
-constexpr int kIRow = 36;       // row stride (floats) of the staged weight rows
-
-template <int PX>
-struct ILds {
-  static constexpr int WZ = 0;                                 // float [kINP][kIRow]
-  static constexpr int WY = WZ + kINP * kIRow * 4;             // float [kINP][kIRow]
-  static constexpr int WX = WY + kINP * kIRow * 4;             // float [kINP][kIRow]
-  static constexpr int RED = WX + kINP * kIRow * 4;            // float2 [8][kINP]
-  static constexpr int JIDX = RED + 8 * kINP * 8;              // int [kINP]
-  static constexpr int ZS = JIDX + kINP * 4;                   // float [3][kINP]
-  static constexpr int OFS = ZS + 3 * kINP * 4;                // int [kINP]
-  static constexpr int TMX = OFS + kINP * 4;                   // float [8]
-  static constexpr int POLY = TMX + 8 * 4;                     // float [kMaxTaps][kPolyRow]
-  static constexpr int SIZE = POLY + kMaxTaps * kPolyRow * 4;
-};
-static_assert(2 * ILds<16>::SIZE <= 160 * 1024, "MFMA interpolation LDS");
-
-
-template <int W>
-__global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_mfma_kernel(
-    PointArgs pa, ZRec zr, const float2 *grid, const float *tmax, float2 *out, Geom g,
-    const KbPoly *__restrict__ dpoly) {
-  using MS = MSpread<W>;
-  constexpr int NT = MS::NT, XW = MS::XW, PX = MS::PX, NW = MS::NW;
-  using L = ILds<PX>;
-  extern __shared__ __align__(16) char shm[];
-  float *WZ = (float *)(shm + L::WZ);
-  float *WY = (float *)(shm + L::WY);
-  float *WX = (float *)(shm + L::WX);
-  float2 *RED = (float2 *)(shm + L::RED);
-  int *JIDX = (int *)(shm + L::JIDX);
-  float *ZS = (float *)(shm + L::ZS);
-  int *OFS = (int *)(shm + L::OFS);
-  float *PL = (float *)(shm + L::POLY);
-  stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first round's barriers
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int h = lane >> 5, l32 = lane & 31;
-  const XcdChunk xr(pa);
-  for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
-    const int4 sub = pa.subs[sidx];
-    const int bin = sub.x;
-    const int bz = bin % g.nbin[2];
-    const int by = (bin / g.nbin[2]) % g.nbin[1];
-    const int bx = bin / (g.nbin[2] * g.nbin[1]);
-    const int bx0 = bx * g.bin[0];
-    const int oy = by * g.bin[1] - W, oz = bz * g.bin[2] - W;
-
-    // A operands: lane (l32, h) holds G[x][y = l32][z] for z = 8h + j (step
-    // 0) and 16 + 8h + j (step 1) of its XW x cells; columns outside the
-    // padded tile are zero.
-    // tile scale from the precomputed tile maximum (tile_max_kernel)
-    const float mx = tmax[bin];
-    int e = 0;
-    if (mx > 0.f) (void)frexpf(mx, &e);
-    e = max(-120, min(120, e));
-    const float s_in = ldexpf(1.0f, -e) * kOpScale, s_out = ldexpf(1.0f, e) * kProdUnscale;
-    const bool yok = l32 < g.pad[1];
-    const int gy = wrap(oy + l32, g.m[1]);
-    half8 arh[XW][2], arl[XW][2], aih[XW][2], ail[XW][2];
-#pragma unroll
-    for (int q = 0; q < XW; ++q) {
-      const int gx = wrap(bx0 - W + 1 + wave * XW + q, g.m[0]);
-      const float2 *rowp = grid + ((int64_t)gx * g.m[1] + gy) * g.m[2];
-#pragma unroll
-      for (int st = 0; st < 2; ++st) {
-        // 8 consecutive z columns: 16-byte loads unless the run wraps or
-        // leaves the padded tile
-        const int zc = 16 * st + 8 * h;
-        const int z0 = oz + zc;
-        float2 v[8];
-        const int64_t e0 = ((int64_t)gx * g.m[1] + gy) * g.m[2] + z0;  // element index
-        if (yok && zc + 8 <= g.pad[2] && z0 >= 0 && z0 + 8 <= g.m[2] && (e0 & 1) == 0) {
-          const float4 *src = (const float4 *)(rowp + z0);
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const float4 t = src[j];
-            v[2 * j] = make_float2(t.x, t.y);
-            v[2 * j + 1] = make_float2(t.z, t.w);
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < 8; ++j)
-            v[j] = (yok && zc + j < g.pad[2]) ? rowp[wrap(z0 + j, g.m[2])] : make_float2(0.f, 0.f);
-        }
-        float tr[8], ti[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          tr[j] = v[j].x * s_in;
-          ti[j] = v[j].y * s_in;
-        }
-        split8(tr, arh[q][st], arl[q][st]);
-        split8(ti, aih[q][st], ail[q][st]);
-      }
-    }
-
-    for (int c0 = 0; c0 < sub.z; c0 += kINP) {
-      const int npts = min(kINP, sub.z - c0);
-      const int ng = (npts + 31) >> 5;
-      const int np32 = ng << 5;
-      const int start = sub.y + c0;
-      __syncthreads();  // the previous round's staged rows and RED are consumed
-      // pass 0: zero the staged rows; arguments, offsets, output indices
-      for (int p = threadIdx.x; p < np32; p += NT) {
-#pragma unroll
-        for (int i = 0; i < 32; i += 4) {
-          *(float4 *)(WX + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-          *(float4 *)(WY + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-          *(float4 *)(WZ + p * kIRow + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        const bool ok = p < npts;
-        const float4 z = ok ? zr.r[2 * (int64_t)(start + p)] : make_float4(0.f, 0.f, 0.f, 0.f);
-        ZS[p] = z.x;
-        ZS[kINP + p] = z.y;
-        ZS[2 * kINP + p] = z.z;
-        OFS[p] = __float_as_int(z.w);
-        if (ok) JIDX[p] = (int)__float_as_uint(zr.r[2 * (int64_t)(start + p) + 1].w);
-      }
-      __syncthreads();
-      // pass 1: tap weights; a task evaluates all taps of one axis for two
-      // points (p, p + np32 / 2)
-      {
-        const int hn = np32 >> 1;
-        for (int u = threadIdx.x; u < 3 * hn; u += NT) {
-          const int a = (u >= hn) + (u >= 2 * hn);
-          const int p0 = u - a * hn, p1 = p0 + hn;
-          const int o0 = (OFS[p0] >> (8 * a)) & 255, o1 = (OFS[p1] >> (8 * a)) & 255;
-          float v0[2 * W], v1[2 * W];
-          taps2<W>(PL, ZS[a * kINP + p0], ZS[a * kINP + p1], v0, v1);
-          float *tab = a == 0 ? WX : (a == 1 ? WY : WZ);
-#pragma unroll
-          for (int t = 0; t < 2 * W; ++t) {
-            tab[p0 * kIRow + o0 + t] = v0[t];
-            tab[p1 * kIRow + o1 + t] = v1[t];
-          }
-        }
-      }
-      __syncthreads();
-      for (int gi = 0; gi < ng; ++gi) {
-        const int k = (gi << 5) + l32;  // this lane's point
-        half8 bh[2], bl[2];
-#pragma unroll
-        for (int st = 0; st < 2; ++st) {
-          float wz[8];
-          *(float4 *)wz = *(const float4 *)(WZ + k * kIRow + 16 * st + 8 * h);
-          *(float4 *)(wz + 4) = *(const float4 *)(WZ + k * kIRow + 16 * st + 8 * h + 4);
-#pragma unroll
-          for (int j = 0; j < 8; ++j) wz[j] *= kOpScale;
-          split8(wz, bh[st], bl[st]);
-        }
-        // wy at the D rows of this lane: y = 8 (r / 4) + 4 h + r % 4
-        float wy[16];
-#pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4)
-          *(float4 *)(wy + 4 * r4) = *(const float4 *)(WY + k * kIRow + 8 * r4 + 4 * h);
-        float wx[XW];
-#pragma unroll
-        for (int q = 0; q < XW; ++q) wx[q] = WX[k * kIRow + wave * XW + q];
-        float accr = 0.f, acci = 0.f;
-#pragma unroll
-        for (int q = 0; q < XW; ++q) {
-          f32x16 dr, di;
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            dr[r] = 0.f;
-            di[r] = 0.f;
-          }
-#pragma unroll
-          for (int st = 0; st < 2; ++st) {
-            dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bh[st], dr, 0, 0, 0);
-            dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arh[q][st], bl[st], dr, 0, 0, 0);
-            dr = __builtin_amdgcn_mfma_f32_32x32x16_f16(arl[q][st], bh[st], dr, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bh[st], di, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f32_32x32x16_f16(aih[q][st], bl[st], di, 0, 0, 0);
-            di = __builtin_amdgcn_mfma_f32_32x32x16_f16(ail[q][st], bh[st], di, 0, 0, 0);
-          }
-          float pr = 0.f, pi = 0.f;
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            pr = fmaf(wy[r], dr[r], pr);
-            pi = fmaf(wy[r], di[r], pi);
-          }
-          accr = fmaf(wx[q], pr, accr);
-          acci = fmaf(wx[q], pi, acci);
-        }
-        accr += __shfl_xor(accr, 32);
-        acci += __shfl_xor(acci, 32);
-        if (h == 0) RED[wave * kINP + k] = make_float2(accr, acci);
-      }
-      __syncthreads();
-      for (int p = threadIdx.x; p < npts; p += NT) {
-        float2 v = RED[p];
-#pragma unroll
-        for (int w2 = 1; w2 < NW; ++w2) {
-          const float2 u = RED[w2 * kINP + p];
-          v.x += u.x;
-          v.y += u.y;
-        }
-        out[JIDX[p]] = make_float2(v.x * s_out, v.y * s_out);
-      }
-    }
-  }
-}
-
-// ---- Windowed MFMA interpolation (2w <= 14) --------------------------------
-//
-// A point's 2w tap rows in y start at its tile offset o1 in [1, 32 - 2w] (the
-// binning's record offset), so a 32 x 32 product block uses 2w of its 32 y
-// rows.  Points whose o1 lie in one class of 17 - 2w consecutive offsets all
-// fit one 16-row window of the tile, starting at row origin(c); for such a
-// group of 16 points the y contraction runs on v_mfma_f32_16x16x32_f16 (16
-// window rows x 16 points, K = the tile's 32 z columns): half the matrix work
-// per point of the 32 x 32 form.
+// Same tiles as MFMA spreading; the tile is scaled by a power of two so that
+// its largest component is in [1/2, 1) (from the tile maxima) and split into
+// FP16 hi + lo like the spreading operands.  A point's 2w tap rows in y start
+// at its tile offset o1 in [1, 32 - 2w] (the binning's record offset), so a
+// 32 x 32 product block would use 2w of its 32 y rows.  Points whose o1 lie
+// in one class of 17 - 2w consecutive offsets (one offset at w = 8) all fit
+// one 16-row window of the tile, starting at row origin(c); for such a group
+// of 16 points the y contraction runs on v_mfma_f32_16x16x32_f16 (16 window
+// rows x 16 points, K = the tile's 32 z columns): half the matrix work per
+// point of the 32 x 32 form.
 //
 // The tile lives in registers as two 16-row halves: lane (i, g) (i = lane %
 // 16, g = lane / 16) holds rows i and 16 + i of the z columns 8g .. 8g + 7,
@@ -1240,21 +1030,27 @@ constexpr int kJRowB = 40;     // row stride (halves) of the staged B operands (
 constexpr int kJRowY = 20;     // row stride (floats) of the y weights (16 window rows)
 constexpr int kJRowX = 20;     // of the x weights (16 columns)
 constexpr int kJPiece = 4096;  // points listed by class at once
+constexpr int kMaxCls = 16;    // classes at most (w = 8: one per offset)
+constexpr int kClsInts = 3 * kMaxCls + 4;  // counts, cursors, starts (kMaxCls + 1), padding
+constexpr int kListLen = kJPiece + 16 * kMaxCls;  // list entries with each class's padding
+template <int PX, int NW>
 struct JLds {
+  static constexpr int ROWX = PX + 4;                   // row stride (floats) of the x weights
   static constexpr int BH = 0;                          // half [kJNP][kJRowB]: hi of z weights
   static constexpr int BL = BH + kJNP * kJRowB * 2;     // half [kJNP][kJRowB]: lo
   static constexpr int WY = BL + kJNP * kJRowB * 2;     // float [kJNP][kJRowY]
-  static constexpr int WX = WY + kJNP * kJRowY * 4;     // float [kJNP][kJRowX]
-  static constexpr int RED = WX + kJNP * kJRowX * 4;    // float2 [16][kJNP]: per wave, lane row
-  static constexpr int JIDX = RED + 16 * kJNP * 8;      // int [kJNP]
+  static constexpr int WX = WY + kJNP * kJRowY * 4;     // float [kJNP][ROWX]
+  static constexpr int RED = WX + kJNP * ROWX * 4;      // float2 [4 NW][kJNP]: per wave, lane row
+  static constexpr int JIDX = RED + 4 * NW * kJNP * 8;  // int [kJNP]
   static constexpr int ZS = JIDX + kJNP * 4;            // float [3][kJNP]
   static constexpr int OFS = ZS + 3 * kJNP * 4;         // int [kJNP]
-  static constexpr int CNT = OFS + kJNP * 4;            // int [8] counts, [8] cursors, [8] starts
-  static constexpr int LIST = CNT + 24 * 4;             // u16 [kJPiece + 16 * 8]
-  static constexpr int POLY = LIST + (kJPiece + 128) * 2;  // float [kMaxTaps][kPolyRow]
+  static constexpr int CNT = OFS + kJNP * 4;            // int [kClsInts]
+  static constexpr int LIST = CNT + kClsInts * 4;       // u16 [kListLen]
+  static constexpr int POLY = LIST + kListLen * 2;      // float [kMaxTaps][kPolyRow]
   static constexpr int SIZE = POLY + kMaxTaps * kPolyRow * 4;
 };
-static_assert(2 * JLds::SIZE <= 160 * 1024, "windowed interpolation LDS");
+static_assert(2 * JLds<16, 4>::SIZE <= 160 * 1024, "windowed interpolation: two workgroups per CU");
+static_assert(JLds<32, 8>::SIZE <= 160 * 1024, "windowed interpolation at w = 8: one workgroup per CU");
 static_assert(kJPiece < 0xFFFF, "list entries are u16 with 0xFFFF for padding");
 
 typedef int int4v __attribute__((ext_vector_type(4)));
@@ -1270,28 +1066,36 @@ __device__ __forceinline__ half8 window_rows(const half8 &lower, const half8 &up
   return __builtin_bit_cast(half8, r);
 }
 
+// Workgroups per CU of the windowed interpolation: two 4-wave groups for
+// w <= 7 (16 x cells), one 8-wave group at w = 8 (32 x cells); 256
+// registers per wave either way.
 template <int W>
-__global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_win_kernel(
+constexpr int win_groups_per_cu() { return W <= 7 ? 2 : 1; }
+
+template <int W>
+__global__ __launch_bounds__(MSpread<W>::NT, win_groups_per_cu<W>()) void interp3d_win_kernel(
     PointArgs pa, ZRec zr, const float2 *grid, const float *tmax, float2 *out, Geom g,
     const KbPoly *__restrict__ dpoly) {
   using MS = MSpread<W>;
   using MW = MWin<W>;
-  constexpr int NT = MS::NT, XW = MS::XW, NW = MS::NW, NCLS = MW::NCLS;
-  static_assert(NW == 4 && XW == 4 && MS::PX == 16 && NCLS <= 8, "windowed interpolation shape");
+  constexpr int NT = MS::NT, XW = MS::XW, NW = MS::NW, PX = MS::PX, NCLS = MW::NCLS;
+  static_assert(XW == 4 && PX == 4 * NW && NCLS <= kMaxCls, "windowed interpolation shape");
+  using L = JLds<PX, NW>;
+  constexpr int ROWX = L::ROWX;
   extern __shared__ __align__(16) char shm[];
-  _Float16 *BH = (_Float16 *)(shm + JLds::BH);
-  _Float16 *BL = (_Float16 *)(shm + JLds::BL);
-  float *WY = (float *)(shm + JLds::WY);
-  float *WX = (float *)(shm + JLds::WX);
-  float2 *RED = (float2 *)(shm + JLds::RED);
-  int *JIDX = (int *)(shm + JLds::JIDX);
-  float *ZS = (float *)(shm + JLds::ZS);
-  int *OFS = (int *)(shm + JLds::OFS);
-  int *CNT = (int *)(shm + JLds::CNT);
-  int *CUR = CNT + 8;
-  int *START = CNT + 16;
-  uint16_t *LIST = (uint16_t *)(shm + JLds::LIST);
-  float *PL = (float *)(shm + JLds::POLY);
+  _Float16 *BH = (_Float16 *)(shm + L::BH);
+  _Float16 *BL = (_Float16 *)(shm + L::BL);
+  float *WY = (float *)(shm + L::WY);
+  float *WX = (float *)(shm + L::WX);
+  float2 *RED = (float2 *)(shm + L::RED);
+  int *JIDX = (int *)(shm + L::JIDX);
+  float *ZS = (float *)(shm + L::ZS);
+  int *OFS = (int *)(shm + L::OFS);
+  int *CNT = (int *)(shm + L::CNT);
+  int *CUR = CNT + kMaxCls;
+  int *START = CNT + 2 * kMaxCls;
+  uint16_t *LIST = (uint16_t *)(shm + L::LIST);
+  float *PL = (float *)(shm + L::POLY);
   stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first piece's barriers
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1426,10 +1230,9 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_win_kernel(
             *(float4 *)(BL + s * kJRowB + i) = make_float4(0.f, 0.f, 0.f, 0.f);
           }
 #pragma unroll
-          for (int i = 0; i < 16; i += 4) {
-            *(float4 *)(WY + s * kJRowY + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-            *(float4 *)(WX + s * kJRowX + i) = make_float4(0.f, 0.f, 0.f, 0.f);
-          }
+          for (int i = 0; i < 16; i += 4) *(float4 *)(WY + s * kJRowY + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int i = 0; i < PX; i += 4) *(float4 *)(WX + s * ROWX + i) = make_float4(0.f, 0.f, 0.f, 0.f);
           const float4 z = ok ? zr.r[2 * (base + li)] : make_float4(0.f, 0.f, 0.f, 0.f);
           ZS[s] = z.x;
           ZS[kJNP + s] = z.y;
@@ -1462,7 +1265,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_win_kernel(
               }
             } else {
               float *tab = a == 0 ? WX : WY;
-              const int stride = a == 0 ? kJRowX : kJRowY;
+              const int stride = a == 0 ? ROWX : kJRowY;
 #pragma unroll
               for (int t = 0; t < 2 * W; ++t) {
                 tab[s0 * stride + o0 + t] = v0[t];
@@ -1494,7 +1297,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void interp3d_win_kernel(
           const half8 bl = *(const half8 *)(BL + k * kJRowB + 8 * g4);
           // wy at this lane's D rows (window rows 4 g4 .. 4 g4 + 3), wx at the wave's cells
           const float4 wy = *(const float4 *)(WY + k * kJRowY + 4 * g4);
-          const float4 wx4 = *(const float4 *)(WX + k * kJRowX + wave * XW);
+          const float4 wx4 = *(const float4 *)(WX + k * ROWX + wave * XW);
           const float wx[4] = {wx4.x, wx4.y, wx4.z, wx4.w};
           float accr = 0.f, acci = 0.f;
           // products of cell q + 1 are issued before the contraction of cell q
@@ -1584,9 +1387,9 @@ struct GJLds {
   static constexpr int ZS = U + kGJNP * 8;                 // float [3][kGJNP]
   static constexpr int OFS = ZS + 3 * kGJNP * 4;           // int [kGJNP]
   static constexpr int TMX = OFS + kGJNP * 4;              // float [16]
-  static constexpr int CNT = TMX + 16 * 4;                 // int [8] counts, [8] cursors, [8] starts
-  static constexpr int LIST = CNT + 24 * 4;                // u16 [kJPiece + 16 * 8]
-  static constexpr int POLY = LIST + (kJPiece + 128) * 2;  // float [2][kMaxTaps][kPolyRow]
+  static constexpr int CNT = TMX + 16 * 4;                 // int [kClsInts]
+  static constexpr int LIST = CNT + kClsInts * 4;          // u16 [kListLen]
+  static constexpr int POLY = LIST + kListLen * 2;         // float [2][kMaxTaps][kPolyRow]
   static constexpr int SIZE = POLY + 2 * kMaxTaps * kPolyRow * 4;
   static_assert(SIZE <= 160 * 1024, "one windowed gradient workgroup per CU");
 };
@@ -1616,7 +1419,7 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
   using MG = MGrad<W>;
   using MW = MWin<W>;
   constexpr int NT = MG::NT, XW = MG::XW, NW = MG::NW, NCLS = MW::NCLS;
-  static_assert(XW == 2 && MG::PX == 16 && NCLS <= 8, "windowed gradient shape");
+  static_assert(XW == 2 && MG::PX == 16 && NCLS <= kMaxCls, "windowed gradient shape");
   using L = GJLds<NW>;
   extern __shared__ __align__(16) char shm[];
   _Float16 *BH = (_Float16 *)(shm + L::BH);
@@ -1634,8 +1437,8 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
   int *OFS = (int *)(shm + L::OFS);
   float *tmx = (float *)(shm + L::TMX);
   int *CNT = (int *)(shm + L::CNT);
-  int *CUR = CNT + 8;
-  int *START = CNT + 16;
+  int *CUR = CNT + kMaxCls;
+  int *START = CNT + 2 * kMaxCls;
   uint16_t *LIST = (uint16_t *)(shm + L::LIST);
   float *PL = (float *)(shm + L::POLY);
   float *PLD = PL + kMaxTaps * kPolyRow;
@@ -2288,7 +2091,7 @@ struct SpreadBinX {
 template <int W>
 struct MSpreadLds {
   static hipError_t run(size_t *out) {
-    *out = std::max<size_t>(MLds<MSpread<W>::PX>::SIZE, ILds<MSpread<W>::PX>::SIZE);
+    *out = std::max<size_t>(MLds<MSpread<W>::PX>::SIZE, JLds<MSpread<W>::PX, MSpread<W>::NW>::SIZE);
     return hipSuccess;
   }
 };
@@ -2334,22 +2137,13 @@ struct Interp3Mfma {
   static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
                         float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
     const float *tmax = bmax + g.nbin[0] * g.nbin[1] * g.nbin[2];  // from TileMax3
-    if constexpr (W <= 7) {
-      // 16-row windows hold the 2w tap rows of a class of points
-      constexpr int lds = JLds::SIZE;
-      hipError_t e = hipFuncSetAttribute((const void *)interp3d_win_kernel<W>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(interp3d_win_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                         zr, grid, (const float *)tmax, out, g, dpoly);
-    } else {
-      constexpr int lds = ILds<MSpread<W>::PX>::SIZE;
-      hipError_t e = hipFuncSetAttribute((const void *)interp3d_mfma_kernel<W>,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-      if (e != hipSuccess) return e;
-      hipLaunchKernelGGL(interp3d_mfma_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa,
-                         zr, grid, (const float *)tmax, out, g, dpoly);
-    }
+    // 16-row windows hold the 2w tap rows of a class of points
+    constexpr int lds = JLds<MSpread<W>::PX, MSpread<W>::NW>::SIZE;
+    hipError_t e = hipFuncSetAttribute((const void *)interp3d_win_kernel<W>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(interp3d_win_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa, zr,
+                       grid, (const float *)tmax, out, g, dpoly);
     return hipGetLastError();
   }
 };
@@ -2450,6 +2244,11 @@ int spread_bin_x(int w, int method) {
   int v = kSpreadBinX;
   (void)dispatch_w<SpreadBinX>(w, method, &v);
   return v;
+}
+
+int interp_blocks_per_cu(int w, int method) {
+  if (method != kSpreadMfma) return 4;
+  return w <= 7 ? 2 : 1;  // win_groups_per_cu
 }
 
 int spread_block_threads(int w, int method) {
