@@ -190,6 +190,94 @@ __device__ __forceinline__ void run_stages(const FftPass &p, int Lf, const float
   }
 }
 
+// Elements of a pass handled per thread at a time: their global loads (the
+// input, its deconvolution factor) are all issued before any is used.  Past
+// the end a lane loads element 0 of its block and drops it, so no branch
+// around a load makes the compiler wait for it there.
+constexpr int kLoadBatch = 4;
+
+// (line, index) of element idx of a block: index-major when the lines are
+// strided (adjacent lines adjacent in memory), line-major otherwise.
+__device__ __forceinline__ int2 line_index(int idx, bool contig, int n, uint32_t mag_n, int nl,
+                                           uint32_t mag_nl) {
+  if (contig) {
+    const int li = fdiv(idx, n, mag_n);
+    return make_int2(li, idx - li * n);
+  }
+  const int i = fdiv(idx, nl, mag_nl);
+  return make_int2(idx - i * nl, i);
+}
+
+// Input lines into LDS: deconvolution (DEC), conjugation for the inverse,
+// the Bluestein chirp, kept modes at their FFT positions.
+template <bool DEC>
+__device__ __forceinline__ void load_lines(const FftPass &p, const int64_t *lbase, int nl,
+                                           uint32_t mag_nl, int Lf, bool blue, float2 *buf0) {
+  const bool contig = p.in_si == 1;
+  const int n = nl * p.nin;
+  for (int base = threadIdx.x; base < n; base += kFftThreads * kLoadBatch) {
+    float2 v[kLoadBatch];
+    double f[kLoadBatch];
+    int2 e[kLoadBatch];  // (line, index)
+#pragma unroll
+    for (int r = 0; r < kLoadBatch; ++r) {
+      const int idx = base + r * kFftThreads;
+      e[r] = line_index(idx < n ? idx : 0, contig, p.nin, p.mag_nin, nl, mag_nl);
+      v[r] = p.in[lbase[e[r].x] + (int64_t)e[r].y * p.in_si];
+      f[r] = DEC ? p.dec_in[e[r].y] : 1.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kLoadBatch; ++r) {
+      float2 x = v[r];
+      if (DEC) {
+        const float s = (float)f[r];
+        x.x *= s;
+        x.y *= s;
+      }
+      if (p.inverse) x.y = -x.y;
+      const int pos = kept_pos(e[r].y, p.nin, p.L);
+      if (blue) x = cmul(x, p.chirp[pos]);
+      if (base + r * kFftThreads < n) buf0[e[r].x * Lf + pos] = x;
+    }
+  }
+}
+
+// LDS lines to the output: kept modes, Bluestein chirp, conjugation for the
+// inverse, scale and deconvolution (DEC), and the bin maxima (bm != null).
+template <bool DEC>
+__device__ __forceinline__ void store_lines(const FftPass &p, const int64_t *lbase, int nl,
+                                            uint32_t mag_nl, int Lf, bool blue, const float2 *src,
+                                            uint32_t *bm, int nbz) {
+  const bool contig = p.out_si == 1;
+  const int n = nl * p.nout;
+  for (int base = threadIdx.x; base < n; base += kFftThreads * kLoadBatch) {
+    double f[kLoadBatch];
+    int2 e[kLoadBatch];  // (line, output index)
+#pragma unroll
+    for (int r = 0; r < kLoadBatch; ++r) {
+      const int idx = base + r * kFftThreads;
+      e[r] = line_index(idx < n ? idx : 0, contig, p.nout, p.mag_nout, nl, mag_nl);
+      f[r] = DEC ? p.dec_out[e[r].y] : 1.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kLoadBatch; ++r) {
+      if (base + r * kFftThreads >= n) continue;
+      const int li = e[r].x, o = e[r].y;
+      const int pos = kept_pos(o, p.nout, p.L);
+      float2 v = src[li * Lf + pos];
+      if (blue) v = cmul(make_float2(v.x, -v.y), p.chirp[pos]);  // X_k = chirp_k conj(c_k)
+      if (p.inverse) v.y = -v.y;
+      const float s = (float)(DEC ? p.scale * f[r] : p.scale);
+      v.x *= s;
+      v.y *= s;
+      p.out[lbase[p.lb + li] + (int64_t)o * p.out_si] = v;
+      if (bm)
+        atomicMax(&bm[li * nbz + fdiv(o, p.bm_bin[2], p.mag_bmz)],
+                  __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
+    }
+  }
+}
+
 __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   extern __shared__ float2 sh[];
   const int L = p.L, lb = p.lb;
@@ -199,7 +287,10 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   float2 *buf1 = sh + (size_t)lb * Lf;
   float2 *tw = sh + 2 * (size_t)lb * Lf;  // twiddle table of length Lf
   int64_t *lbase = (int64_t *)(tw + Lf);  // [lb] input, [lb] output line offsets
-  for (int i = threadIdx.x; i < Lf; i += kFftThreads) tw[i] = p.tw[i];
+  // the first 2 x kFftThreads twiddles: loaded now, stored to LDS after the
+  // input lines (their latency overlaps the line set-up and loads)
+  const float2 tw0 = p.tw[min((int)threadIdx.x, Lf - 1)];
+  const float2 tw1 = p.tw[min((int)threadIdx.x + kFftThreads, Lf - 1)];
   if (p.zero)
     for (int i = blockIdx.x * kFftThreads + threadIdx.x; i < p.zero_n; i += gridDim.x * kFftThreads)
       p.zero[i] = 0.f;
@@ -220,27 +311,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     __syncthreads();
   }
   // ---- load (with deconvolution, conjugation for the inverse, zero pad) ----
-  const bool in_contig = p.in_si == 1;
-  for (int idx = threadIdx.x; idx < nl * p.nin; idx += kFftThreads) {
-    int li, i;
-    if (in_contig) {
-      li = fdiv(idx, p.nin, p.mag_nin);
-      i = idx - li * p.nin;
-    } else {
-      i = fdiv(idx, nl, mag_nl);
-      li = idx - i * nl;
-    }
-    float2 v = p.in[lbase[li] + (int64_t)i * p.in_si];
-    if (p.dec_in) {
-      float f = (float)p.dec_in[i];
-      v.x *= f;
-      v.y *= f;
-    }
-    if (p.inverse) v.y = -v.y;
-    const int pos = kept_pos(i, p.nin, L);
-    if (blue) v = cmul(v, p.chirp[pos]);
-    buf0[li * Lf + pos] = v;
-  }
+  if (p.dec_in) load_lines<true>(p, lbase, nl, mag_nl, Lf, blue, buf0);
+  else load_lines<false>(p, lbase, nl, mag_nl, Lf, blue, buf0);
+  if ((int)threadIdx.x < Lf) tw[threadIdx.x] = tw0;
+  if ((int)threadIdx.x + kFftThreads < Lf) tw[threadIdx.x + kFftThreads] = tw1;
+  for (int i = threadIdx.x + 2 * kFftThreads; i < Lf; i += kFftThreads) tw[i] = p.tw[i];
   if (!blue && p.nin < L) {
     const int gap = L - p.nin, g0 = p.nin / 2 + 1;
     for (int idx = threadIdx.x; idx < nl * gap; idx += kFftThreads) {
@@ -273,27 +348,9 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
     for (int i = threadIdx.x; i < nl * nbz; i += kFftThreads) bm[i] = 0u;
     __syncthreads();
   }
-  const bool out_contig = p.out_si == 1;
-  for (int idx = threadIdx.x; idx < nl * p.nout; idx += kFftThreads) {
-    int li, o;
-    if (out_contig) {
-      li = fdiv(idx, p.nout, p.mag_nout);
-      o = idx - li * p.nout;
-    } else {
-      o = fdiv(idx, nl, mag_nl);
-      li = idx - o * nl;
-    }
-    const int pos = kept_pos(o, p.nout, L);
-    float2 v = src[li * Lf + pos];
-    if (blue) v = cmul(make_float2(v.x, -v.y), p.chirp[pos]);  // X_k = chirp_k conj(c_k)
-    if (p.inverse) v.y = -v.y;
-    float f = (float)(p.dec_out ? p.scale * p.dec_out[o] : p.scale);
-    v.x *= f;
-    v.y *= f;
-    p.out[lbase[lb + li] + (int64_t)o * p.out_si] = v;
-    if (p.bmax)
-      atomicMax(&bm[li * nbz + fdiv(o, p.bm_bin[2], p.mag_bmz)], __float_as_uint(fmaxf(fabsf(v.x), fabsf(v.y))));
-  }
+  uint32_t *bmx = p.bmax ? bm : nullptr;
+  if (p.dec_out) store_lines<true>(p, lbase, nl, mag_nl, Lf, blue, src, bmx, nbz);
+  else store_lines<false>(p, lbase, nl, mag_nl, Lf, blue, src, bmx, nbz);
   if (p.bmax) {
     __syncthreads();
     for (int i = threadIdx.x; i < nl * nbz; i += kFftThreads) {
