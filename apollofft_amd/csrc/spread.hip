@@ -481,10 +481,19 @@ __device__ __forceinline__ MTile mfma_tile(const Geom &g, int bin) {
 template <int NW>
 __device__ __forceinline__ int mfma_strength_exp(const ZRec &zr, int4 sub, float *red) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int NT = 64 * NW, B = 4;  // B values per thread loaded before any is used
+  const float2 *vals = (const float2 *)zr.r + 4 * (int64_t)sub.y + 2;  // point p's value: vals[4 p]
   float mx = 0.f;
-  for (int p = threadIdx.x; p < sub.z; p += 64 * NW) {
-    const float4 c = zr.r[2 * (int64_t)(sub.y + p) + 1];
-    mx = fmaxf(mx, fmaxf(fabsf(c.x), fabsf(c.y)));
+  for (int p0 = threadIdx.x; p0 < sub.z; p0 += NT * B) {
+    float2 c[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int p = p0 + u * NT;
+      c[u] = vals[4 * (int64_t)(p < sub.z ? p : 0)];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (p0 + u * NT < sub.z) mx = fmaxf(mx, fmaxf(fabsf(c[u].x), fabsf(c[u].y)));
   }
   mx = wave_max_nonneg(mx);
   __syncthreads();  // earlier readers of red[] are done
@@ -1046,7 +1055,8 @@ struct JLds {
   static constexpr int OFS = ZS + 3 * kJNP * 4;         // int [kJNP]
   static constexpr int CNT = OFS + kJNP * 4;            // int [kClsInts]
   static constexpr int LIST = CNT + kClsInts * 4;       // u16 [kListLen]
-  static constexpr int POLY = LIST + kListLen * 2;      // float [kMaxTaps][kPolyRow]
+  static constexpr int CLS = LIST + kListLen * 2;       // u8 [kJPiece]: class of each point
+  static constexpr int POLY = CLS + kJPiece;            // float [kMaxTaps][kPolyRow]
   static constexpr int SIZE = POLY + kMaxTaps * kPolyRow * 4;
 };
 static_assert(2 * JLds<16, 4>::SIZE <= 160 * 1024, "windowed interpolation: two workgroups per CU");
@@ -1069,6 +1079,80 @@ __device__ __forceinline__ half8 window_rows(const half8 &lower, const half8 &up
 // Workgroups per CU of the windowed interpolation: two 4-wave groups for
 // w <= 7 (16 x cells), one 8-wave group at w = 8 (32 x cells); 256
 // registers per wave either way.
+// The class list of one piece of np points (records base .. base + np - 1):
+// the count of each class (one ballot per class and wave), class starts
+// padded to whole groups of 16, each point's local index at its class's
+// cursor and 0xFFFF in the pad slots.  The classes come from the records'
+// offset words, loaded kClsBatch per thread before any is used, and are kept
+// as bytes in CLS for the second pass.  Ends with a barrier.
+constexpr int kClsBatch = 4;
+template <class MW, int NT>
+__device__ __forceinline__ void class_list(const ZRec &zr, int64_t base, int np, int *CNT,
+                                           uint8_t *CLS, uint16_t *LIST) {
+  constexpr int NCLS = MW::NCLS;
+  int *CUR = CNT + kMaxCls;
+  int *START = CNT + 2 * kMaxCls;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float *ofs = (const float *)zr.r + 8 * base + 3;  // record p's offset word: ofs[8 p]
+  if (threadIdx.x < NCLS) CNT[threadIdx.x] = 0;
+  __syncthreads();
+  for (int pw = wave * 64; pw < np; pw += NT * kClsBatch) {
+    int c[kClsBatch];
+#pragma unroll
+    for (int u = 0; u < kClsBatch; ++u) {
+      const int p = pw + u * NT + lane;
+      const int o = __float_as_int(ofs[8 * (int64_t)(p < np ? p : 0)]);
+      c[u] = p < np ? MW::cls((o >> 8) & 255) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < kClsBatch; ++u) {
+      if (pw + u * NT >= np) break;  // wave-uniform
+      const int p = pw + u * NT + lane;
+      if (p < np) CLS[p] = (uint8_t)c[u];
+#pragma unroll
+      for (int k = 0; k < NCLS; ++k) {
+        const uint64_t m = __ballot(c[u] == k);
+        if (lane == 0 && m) atomicAdd(&CNT[k], (int)__popcll(m));
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int k = 0; k < NCLS; ++k) {
+      START[k] = s;
+      CUR[k] = s;
+      s += (CNT[k] + 15) & ~15;
+    }
+    START[NCLS] = s;
+  }
+  __syncthreads();
+  for (int pw = wave * 64; pw < np; pw += NT) {
+    const int p = pw + lane;
+    const int c = p < np ? (int)CLS[p] : -1;
+    int pos = 0;
+#pragma unroll
+    for (int k = 0; k < NCLS; ++k) {
+      const uint64_t m = __ballot(c == k);
+      if (m) {
+        int b = 0;
+        if (lane == 0) b = atomicAdd(&CUR[k], (int)__popcll(m));
+        b = __builtin_amdgcn_readfirstlane(b);
+        const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (c == k) pos = b + rank;
+      }
+    }
+    if (c >= 0) LIST[pos] = (uint16_t)p;
+  }
+  for (int t = threadIdx.x; t < 16 * NCLS; t += NT) {
+    const int k = t >> 4;
+    const int pos = START[k] + CNT[k] + (t & 15);
+    if (pos < START[k + 1]) LIST[pos] = 0xFFFF;
+  }
+  __syncthreads();
+}
+
 template <int W>
 constexpr int win_groups_per_cu() { return W <= 7 ? 2 : 1; }
 
@@ -1092,9 +1176,9 @@ __global__ __launch_bounds__(MSpread<W>::NT, win_groups_per_cu<W>()) void interp
   float *ZS = (float *)(shm + L::ZS);
   int *OFS = (int *)(shm + L::OFS);
   int *CNT = (int *)(shm + L::CNT);
-  int *CUR = CNT + kMaxCls;
   int *START = CNT + 2 * kMaxCls;
   uint16_t *LIST = (uint16_t *)(shm + L::LIST);
+  uint8_t *CLS = (uint8_t *)(shm + L::CLS);
   float *PL = (float *)(shm + L::POLY);
   stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first piece's barriers
 
@@ -1159,54 +1243,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, win_groups_per_cu<W>()) void interp
       const int np = min(kJPiece, sub.z - p0);
       const int64_t base = (int64_t)sub.y + p0;
       __syncthreads();  // the previous piece's list, rows and sums are consumed
-      if (threadIdx.x < NCLS) CNT[threadIdx.x] = 0;
-      __syncthreads();
-      // class counts (one ballot per class and wave)
-      for (int pw = wave * 64; pw < np; pw += NT) {
-        const int p = pw + lane;
-        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
-#pragma unroll
-        for (int k = 0; k < NCLS; ++k) {
-          const uint64_t m = __ballot(c == k);
-          if (lane == 0 && m) atomicAdd(&CNT[k], (int)__popcll(m));
-        }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int s = 0;
-        for (int k = 0; k < NCLS; ++k) {
-          START[k] = s;
-          CUR[k] = s;
-          s += (CNT[k] + 15) & ~15;
-        }
-        START[NCLS] = s;
-      }
-      __syncthreads();
-      // the list: each point at its class's cursor, then the pad slots
-      for (int pw = wave * 64; pw < np; pw += NT) {
-        const int p = pw + lane;
-        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
-        int pos = 0;
-#pragma unroll
-        for (int k = 0; k < NCLS; ++k) {
-          const uint64_t m = __ballot(c == k);
-          if (m) {
-            int b = 0;
-            if (lane == 0) b = atomicAdd(&CUR[k], (int)__popcll(m));
-            b = __builtin_amdgcn_readfirstlane(b);
-            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (c == k) pos = b + rank;
-          }
-        }
-        if (c >= 0) LIST[pos] = (uint16_t)p;
-      }
-      for (int t = threadIdx.x; t < 16 * NCLS; t += NT) {
-        const int k = t >> 4;
-        const int pos = START[k] + CNT[k] + (t & 15);
-        if (pos < START[k + 1]) LIST[pos] = 0xFFFF;
-      }
-      __syncthreads();
+      class_list<MW, NT>(zr, base, np, CNT, CLS, LIST);
       int cstart[NCLS + 1];
 #pragma unroll
       for (int k = 0; k <= NCLS; ++k) cstart[k] = __builtin_amdgcn_readfirstlane(START[k]);
@@ -1389,7 +1426,8 @@ struct GJLds {
   static constexpr int TMX = OFS + kGJNP * 4;              // float [16]
   static constexpr int CNT = TMX + 16 * 4;                 // int [kClsInts]
   static constexpr int LIST = CNT + kClsInts * 4;          // u16 [kListLen]
-  static constexpr int POLY = LIST + kListLen * 2;         // float [2][kMaxTaps][kPolyRow]
+  static constexpr int CLS = LIST + kListLen * 2;          // u8 [kJPiece]: class of each point
+  static constexpr int POLY = CLS + kJPiece;               // float [2][kMaxTaps][kPolyRow]
   static constexpr int SIZE = POLY + 2 * kMaxTaps * kPolyRow * 4;
   static_assert(SIZE <= 160 * 1024, "one windowed gradient workgroup per CU");
 };
@@ -1437,9 +1475,9 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
   int *OFS = (int *)(shm + L::OFS);
   float *tmx = (float *)(shm + L::TMX);
   int *CNT = (int *)(shm + L::CNT);
-  int *CUR = CNT + kMaxCls;
   int *START = CNT + 2 * kMaxCls;
   uint16_t *LIST = (uint16_t *)(shm + L::LIST);
+  uint8_t *CLS = (uint8_t *)(shm + L::CLS);
   float *PL = (float *)(shm + L::POLY);
   float *PLD = PL + kMaxTaps * kPolyRow;
   stage_poly_rows(dpoly->c, 2 * W, PL);  // ordered by the first subproblem's barriers
@@ -1514,52 +1552,7 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
       const int np = min(kJPiece, sub.z - p0);
       const int64_t base = (int64_t)sub.y + p0;
       if (p0 > 0) __syncthreads();  // the previous piece's list, rows and sums are consumed
-      if (threadIdx.x < NCLS) CNT[threadIdx.x] = 0;
-      __syncthreads();
-      for (int pw = wave * 64; pw < np; pw += NT) {
-        const int p = pw + lane;
-        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
-#pragma unroll
-        for (int k = 0; k < NCLS; ++k) {
-          const uint64_t m = __ballot(c == k);
-          if (lane == 0 && m) atomicAdd(&CNT[k], (int)__popcll(m));
-        }
-      }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int s = 0;
-        for (int k = 0; k < NCLS; ++k) {
-          START[k] = s;
-          CUR[k] = s;
-          s += (CNT[k] + 15) & ~15;
-        }
-        START[NCLS] = s;
-      }
-      __syncthreads();
-      for (int pw = wave * 64; pw < np; pw += NT) {
-        const int p = pw + lane;
-        const int c = p < np ? MW::cls((__float_as_int(zr.r[2 * (base + p)].w) >> 8) & 255) : -1;
-        int pos = 0;
-#pragma unroll
-        for (int k = 0; k < NCLS; ++k) {
-          const uint64_t m = __ballot(c == k);
-          if (m) {
-            int b = 0;
-            if (lane == 0) b = atomicAdd(&CUR[k], (int)__popcll(m));
-            b = __builtin_amdgcn_readfirstlane(b);
-            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (c == k) pos = b + rank;
-          }
-        }
-        if (c >= 0) LIST[pos] = (uint16_t)p;
-      }
-      for (int t = threadIdx.x; t < 16 * NCLS; t += NT) {
-        const int k = t >> 4;
-        const int pos = START[k] + CNT[k] + (t & 15);
-        if (pos < START[k + 1]) LIST[pos] = 0xFFFF;
-      }
-      __syncthreads();
+      class_list<MW, NT>(zr, base, np, CNT, CLS, LIST);
       int cstart[NCLS + 1];
 #pragma unroll
       for (int k = 0; k <= NCLS; ++k) cstart[k] = __builtin_amdgcn_readfirstlane(START[k]);
