@@ -554,7 +554,7 @@ def main():
     flops = 4.0 * M * T ** 3
     achieved = flops / (dom_ms * 1e-3) / 1e12
     mfma = r["meta"]["kernel_method"] == "mfma"
-    kname = ("spread3d_mfma_kernel" if dom == "spread" else "interp3d_mfma_kernel") if mfma else \
+    kname = ("spread3d_mfma_kernel" if dom == "spread" else "interp3d_win_kernel") if mfma else \
         ("spread3d_kernel" if dom == "spread" else "interp3d_kernel")
     roof = {"bound": "mfma" if mfma else "valu", "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -562,10 +562,11 @@ def main():
             "traffic": load_traffic(kname), "kernel": kname,
             "kernel_ms": round(dom_ms, 4), "flops_per_launch": flops}
     if mfma:
-        # executed per point and tile x cell: 6 (spread, 16 points) or 12
-        # (interp, 32 points) 32x32x16 FP16 MFMAs of 32768 flops each
+        # executed per point and tile x cell: spreading 6 32x32x16 FP16 MFMAs
+        # of 32768 flops per 16 points; interpolation 6 16x16x32 MFMAs of
+        # 16384 flops per 16 points (one 16-row window, pad slots not counted)
         px = r["meta"]["bin_size"][0] + T - 1
-        ex_flops = 12288.0 * px * M
+        ex_flops = (12288.0 if dom == "spread" else 6144.0) * px * M
         ex = ex_flops / (dom_ms * 1e-3) / 1e12
         roof["compute"] = ("FP32-accurate products as FP16 hi/lo split MFMAs (3 per product); "
                            "achieved = algorithmic FP32 flops (4 (2w)^3 per point), priced at "
