@@ -361,17 +361,26 @@ anufft_status build_tables(anufft_plan *p) {
       p->deconv[a][k] = 1.0 / kb_phi_hat(xi, p->w, p->beta, p->i0b);
     }
   }
-  // per-tap polynomials of phi and d phi/dx on x = -w + i + (z+1)/2
+  // per-tap polynomials of phi and d phi/dx on x = -w + i + (z+1)/2.  Tap
+  // 2w-1-i at -z sits at -x, and phi is even (d phi/dx odd), so the fits of
+  // taps 0 .. w-1 are mirrored: P_{2w-1-i}(z) = P_i(-z), D_{2w-1-i}(z) =
+  // -D_i(-z); the kernels evaluate a tap pair from the even and odd parts.
   const int T = 2 * p->w;
   const double wd = (double)p->w;
   double mono[kPolyCoefs];
-  for (int i = 0; i < T; ++i) {
+  for (int i = 0; i < p->w; ++i) {
     cheb_fit([&](double z) { return kb_phi(-wd + i + 0.5 * (z + 1.0), wd, p->beta, p->i0b); },
              mono);
-    for (int d = 0; d < kPolyCoefs; ++d) p->poly.c[d][i] = (float)mono[d];
+    for (int d = 0; d < kPolyCoefs; ++d) {
+      p->poly.c[d][i] = (float)mono[d];
+      p->poly.c[d][T - 1 - i] = (float)((d & 1) ? -mono[d] : mono[d]);
+    }
     cheb_fit([&](double z) { return kb_dphi(-wd + i + 0.5 * (z + 1.0), wd, p->beta, p->i0b); },
              mono);
-    for (int d = 0; d < kPolyCoefs; ++d) p->poly.dc[d][i] = (float)mono[d];
+    for (int d = 0; d < kPolyCoefs; ++d) {
+      p->poly.dc[d][i] = (float)mono[d];
+      p->poly.dc[d][T - 1 - i] = (float)((d & 1) ? mono[d] : -mono[d]);
+    }
   }
   double err = 0.0;
   for (int i = 0; i < T; ++i)

@@ -202,6 +202,41 @@ __device__ __forceinline__ void taps2(const float *__restrict__ rows, float z0, 
   if (z1 >= 1.0f) v1[2 * W - 1] = 0.f;
 }
 
+// taps2 from the mirrored tables (tap 2W-1-i at z is tap i at -z): each
+// pair from the even part E(z^2) and odd part O(z^2) of tap i, P_i = E + z O,
+// P_{2W-1-i} = E - z O; the same endpoint rule.
+static_assert(kPolyCoefs == 11, "taps2_sym splits degree-10 polynomials into even and odd parts");
+template <int W>
+__device__ __forceinline__ void taps2_sym(const float *__restrict__ rows, float z0, float z1,
+                                          float (&v0)[2 * W], float (&v1)[2 * W]) {
+  const float u0 = z0 * z0, u1 = z1 * z1;
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    float c[kPolyRow];
+#pragma unroll
+    for (int d = 0; d < kPolyRow; d += 4) *(float4 *)(c + d) = *(const float4 *)(rows + i * kPolyRow + d);
+    float e0 = c[10], e1 = c[10], o0 = c[9], o1 = c[9];
+#pragma unroll
+    for (int d = 8; d >= 0; d -= 2) {
+      e0 = fmaf(e0, u0, c[d]);
+      e1 = fmaf(e1, u1, c[d]);
+    }
+#pragma unroll
+    for (int d = 7; d >= 1; d -= 2) {
+      o0 = fmaf(o0, u0, c[d]);
+      o1 = fmaf(o1, u1, c[d]);
+    }
+    v0[i] = fmaf(z0, o0, e0);
+    v1[i] = fmaf(z1, o1, e1);
+    v0[2 * W - 1 - i] = fmaf(-z0, o0, e0);
+    v1[2 * W - 1 - i] = fmaf(-z1, o1, e1);
+  }
+  if (z0 <= -1.0f) v0[0] = 0.f;
+  if (z1 <= -1.0f) v1[0] = 0.f;
+  if (z0 >= 1.0f) v0[2 * W - 1] = 0.f;
+  if (z1 >= 1.0f) v1[2 * W - 1] = 0.f;
+}
+
 // LDS staging of the y and z weights of one point, zero-padded to the 32
 // column positions of the padded tile, so a lane reads its weight at its own
 // column index.
@@ -564,7 +599,7 @@ __device__ __forceinline__ void mfma_accumulate(f32x16 (&acc)[MSpread<W>::XW][2]
         const int p0 = u - a * hn, p1 = p0 + hn;
         const int o0 = (OFS[p0] >> (8 * a)) & 255, o1 = (OFS[p1] >> (8 * a)) & 255;
         float v0[T], v1[T];
-        taps2<W>(PL, ZS[a * kMNP + p0], ZS[a * kMNP + p1], v0, v1);
+        taps2_sym<W>(PL, ZS[a * kMNP + p0], ZS[a * kMNP + p1], v0, v1);
         if (a == 0) {
 #pragma unroll
           for (int i = 0; i < T; ++i) {
