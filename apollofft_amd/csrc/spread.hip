@@ -202,11 +202,12 @@ __device__ __forceinline__ void taps2(const float *__restrict__ rows, float z0, 
   if (z1 >= 1.0f) v1[2 * W - 1] = 0.f;
 }
 
-// taps2 from the mirrored tables (tap 2W-1-i at z is tap i at -z): each
-// pair from the even part E(z^2) and odd part O(z^2) of tap i, P_i = E + z O,
-// P_{2W-1-i} = E - z O; the same endpoint rule.
+// taps2 from the mirrored tables (tap 2W-1-i at z is tap i at -z, negated
+// for the derivative, ODD): each pair from the even part E(z^2) and odd part
+// O(z^2) of tap i, P_i = E + z O, P_{2W-1-i} = E - z O (ODD: z O - E); the
+// same endpoint rule.
 static_assert(kPolyCoefs == 11, "taps2_sym splits degree-10 polynomials into even and odd parts");
-template <int W>
+template <int W, bool ODD = false>
 __device__ __forceinline__ void taps2_sym(const float *__restrict__ rows, float z0, float z1,
                                           float (&v0)[2 * W], float (&v1)[2 * W]) {
   const float u0 = z0 * z0, u1 = z1 * z1;
@@ -228,8 +229,8 @@ __device__ __forceinline__ void taps2_sym(const float *__restrict__ rows, float 
     }
     v0[i] = fmaf(z0, o0, e0);
     v1[i] = fmaf(z1, o1, e1);
-    v0[2 * W - 1 - i] = fmaf(-z0, o0, e0);
-    v1[2 * W - 1 - i] = fmaf(-z1, o1, e1);
+    v0[2 * W - 1 - i] = ODD ? fmaf(z0, o0, -e0) : fmaf(-z0, o0, e0);
+    v1[2 * W - 1 - i] = ODD ? fmaf(z1, o1, -e1) : fmaf(-z1, o1, e1);
   }
   if (z0 <= -1.0f) v0[0] = 0.f;
   if (z1 <= -1.0f) v1[0] = 0.f;
@@ -1631,7 +1632,8 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
             const int s0 = u - ad * hn, s1 = s0 + hn;
             const int o0 = (OFS[s0] >> (8 * a)) & 255, o1 = (OFS[s1] >> (8 * a)) & 255;
             float v0[2 * W], v1[2 * W];
-            taps2<W>(d ? PLD : PL, ZS[a * kGJNP + s0], ZS[a * kGJNP + s1], v0, v1);
+            if (d) taps2_sym<W, true>(PLD, ZS[a * kGJNP + s0], ZS[a * kGJNP + s1], v0, v1);
+            else taps2_sym<W>(PL, ZS[a * kGJNP + s0], ZS[a * kGJNP + s1], v0, v1);
             if (a == 2) {
               store_split_rows<2 * W>(d ? DH : BH, d ? DL : BL, s0 * kJRowB + o0, s1 * kJRowB + o1,
                                       v0, v1, d ? kOpScaleD : kOpScale);
