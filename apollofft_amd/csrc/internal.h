@@ -184,6 +184,7 @@ struct BinProblem {
   const float2 *values;     // record values (Type-1 strengths, gradient u) or null
   int dim, nbins, bits, max_sub, sub_work, nchunks;
   uint32_t chunk0, scan0;
+  int fused;                // one problem: the scatter launch also writes the list
 };
 
 // One problem of a batched spreading launch.

@@ -300,14 +300,25 @@ __device__ __forceinline__ uint64_t block_scan_waves64(uint64_t v, uint64_t *sh,
 // tot == NULL with no points means all tiles are empty.
 constexpr int kSubThreads = 1024;
 constexpr int kSubPer = 4;
-constexpr int kSubRound = kSubThreads * kSubPer;
-__device__ __forceinline__ void subproblems_body(uint32_t *bin_start, const uint32_t *tot,
-                                                 int from_tot, int nbins, int3 nb, int max_sub,
-                                                 int sub_work, uint32_t *nsub, int4 *subs) {
-  __shared__ uint32_t sh[kSubThreads / 64];
-  __shared__ uint64_t sh64[kSubThreads / 64];
-  __shared__ uint32_t first[kSubRound];  // first entry of each tile of the round
-  __shared__ int4 tinfo[kSubRound];      // tile: bin, first point, count, work before it
+// LDS of the list for a block of NT threads (rounds of NT * kSubPer tiles).
+template <int NT>
+struct SubLds {
+  static constexpr int kRound = NT * kSubPer;
+  uint32_t sh[NT / 64];
+  uint64_t sh64[NT / 64];
+  uint32_t first[kRound];  // first entry of each tile of the round
+  int4 tinfo[kRound];      // tile: bin, first point, count, work before it
+};
+template <int NT>
+__device__ __forceinline__ void subproblems_body(SubLds<NT> &L, uint32_t *bin_start,
+                                                 const uint32_t *tot, int from_tot, int nbins,
+                                                 int3 nb, int max_sub, int sub_work,
+                                                 uint32_t *nsub, int4 *subs) {
+  constexpr int kSubRound = SubLds<NT>::kRound;
+  uint32_t *sh = L.sh;
+  uint64_t *sh64 = L.sh64;
+  uint32_t *first = L.first;
+  int4 *tinfo = L.tinfo;
   if (from_tot) {
     uint32_t run = 0;
     for (int base = 0; base <= nbins; base += kSubRound) {
@@ -319,7 +330,7 @@ __device__ __forceinline__ void subproblems_body(uint32_t *bin_start, const uint
         sum += v[i];
       }
       uint32_t total;
-      uint32_t ex = run + block_scan_waves<kSubThreads>(sum, sh, total);
+      uint32_t ex = run + block_scan_waves<NT>(sum, sh, total);
 #pragma unroll
       for (int i = 0; i < kSubPer; ++i) {
         if (b0 + i <= nbins) bin_start[b0 + i] = ex;
@@ -352,7 +363,7 @@ __device__ __forceinline__ void subproblems_body(uint32_t *bin_start, const uint
       sum += ((uint64_t)k[i] << 32) | wk[i];
     }
     uint64_t total64;
-    const uint64_t ex = block_scan_waves64<kSubThreads>(sum, sh64, total64);
+    const uint64_t ex = block_scan_waves64<NT>(sum, sh64, total64);
     uint32_t o = (uint32_t)(ex >> 32), w = carry_w + (uint32_t)ex;
 #pragma unroll
     for (int i = 0; i < kSubPer; ++i) {
@@ -363,7 +374,7 @@ __device__ __forceinline__ void subproblems_body(uint32_t *bin_start, const uint
     }
     __syncthreads();
     const uint32_t total = (uint32_t)(total64 >> 32), total_w = (uint32_t)total64;
-    for (uint32_t t = threadIdx.x; t < total; t += kSubThreads) {
+    for (uint32_t t = threadIdx.x; t < total; t += NT) {
       // owner = last tile with first <= t; it has t - first < its count
       int lo = 0, hi = kSubRound - 1;
       while (lo < hi) {
@@ -391,7 +402,9 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_
                                                                   int nbins, int3 nb, int max_sub,
                                                                   int sub_work, uint32_t *nsub,
                                                                   int4 *subs) {
-  subproblems_body(bin_start, tot, from_tot, nbins, nb, max_sub, sub_work, nsub, subs);
+  __shared__ SubLds<kSubThreads> L;
+  subproblems_body<kSubThreads>(L, bin_start, tot, from_tot, nbins, nb, max_sub, sub_work, nsub,
+                                subs);
 }
 
 // ---- one-pass binning (tile count <= kOneMaxBins) ---------------------------
@@ -567,33 +580,83 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_batched_kernel(const 
   const BinProblem &pr = table[blockIdx.x];
   const int3 nb = pr.dim == 3 ? make_int3(pr.g.nbin[0], pr.g.nbin[1], pr.g.nbin[2])
                               : make_int3(1, 1, 1);
-  subproblems_body(pr.bin_start, pr.pa.num_points > 0 ? pr.tot : nullptr, 1, pr.nbins, nb,
-                   pr.max_sub, pr.sub_work, pr.nsub, pr.subs);
+  __shared__ SubLds<kSubThreads> L;
+  subproblems_body<kSubThreads>(L, pr.bin_start, pr.pa.num_points > 0 ? pr.tot : nullptr, 1,
+                                pr.nbins, nb, pr.max_sub, pr.sub_work, pr.nsub, pr.subs);
 }
 
+// Fused list (one problem, one.fused): block 0 writes bin_start and the
+// subproblem list from the tile totals, beside the scatter; every other
+// block scans the totals into LDS for the tile starts of its own points.
+// Otherwise bin_start comes from the list launch before this one.
+// Dynamic LDS: kOneWaves rows of nbins 16-bit counts, then (fused) nbins
+// 32-bit tile starts; block 0 uses it as a SubLds<kOneThreads>.
+constexpr int kFusedTotPer = 16;  // tile totals per thread in the fused scan
+// Largest tile count of the fused list: 16 + 4 bytes of LDS per tile within
+// the 160 KiB of a workgroup (and kFusedTotPer totals per thread).
+constexpr int kFusedMaxBins = 8176;
+static_assert(kFusedMaxBins <= kFusedTotPer * kOneThreads, "fused scan covers the tiles");
+static_assert(20 * kFusedMaxBins + 4 * kOneWaves <= 160 * 1024, "fused scatter LDS");
+static_assert(sizeof(SubLds<kOneThreads>) + 4 * kOneWaves <= 160 * 1024, "list LDS");
 template <bool REC, bool BATCH>
 __global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(BinProblem one,
                                                                     const BinProblem *table,
                                                                     int nprob) {
-  extern __shared__ uint16_t rows[];  // [kOneWaves][nbins]: per-wave tile counts, then offsets
+  extern __shared__ uint4 scatter_lds[];
+  __shared__ uint32_t wsum[kOneWaves];
+  uint16_t *rows = (uint16_t *)scatter_lds;
+  const bool fused = !BATCH && one.fused;
+  if (fused && blockIdx.x == 0) {
+    const int3 nb = one.dim == 3 ? make_int3(one.g.nbin[0], one.g.nbin[1], one.g.nbin[2])
+                                 : make_int3(1, 1, 1);
+    subproblems_body<kOneThreads>(*(SubLds<kOneThreads> *)scatter_lds, one.bin_start, one.tot, 1,
+                                  one.nbins, nb, one.max_sub, one.sub_work, one.nsub, one.subs);
+    return;
+  }
   uint32_t chunk;
   const BinProblem &pr = bin_problem<BATCH, false>(one, table, nprob, chunk);
+  if (fused) chunk -= 1;
   const int nbins = pr.nbins, dim = pr.dim, bits = pr.bits;
   const PointArgs &pa = pr.pa;
   for (int i = threadIdx.x; i < kOneWaves / 2 * nbins; i += kOneThreads) ((uint32_t *)rows)[i] = 0u;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint16_t *row = rows + wave * nbins;
+  uint32_t *bst = (uint32_t *)(rows + kOneWaves * nbins);
   const int64_t j0 = (int64_t)chunk * kOneChunk + (int64_t)wave * (64 * kOneSteps) + lane;
-  // the wave's coordinates, all loads in flight at once
+  // the wave's coordinates and values, all loads in flight at once
   float c[kOneSteps][3];
+  float2 val[kOneSteps];
 #pragma unroll
   for (int s = 0; s < kOneSteps; ++s) {
     const int64_t j = j0 + 64 * s;
-    load_point(pa, dim, j, j < pa.num_points, c[s]);
+    const bool valid = j < pa.num_points;
+    load_point(pa, dim, j, valid, c[s]);
+    val[s] = (REC && pr.values) ? pr.values[valid ? j : 0] : make_float2(0.f, 0.f);
   }
-  __syncthreads();  // rows zeroed
-  uint32_t key[kOneSteps], rk[kOneSteps];
+  // fused: this thread's run of tile totals (clamped indices, no branch)
+  const int per = (nbins + kOneThreads - 1) / kOneThreads;
+  const int t0 = threadIdx.x * per;
+  uint32_t tv[kFusedTotPer];
+  if (fused) {
+#pragma unroll
+    for (int i = 0; i < kFusedTotPer; ++i) tv[i] = pr.tot[min(t0 + i, nbins - 1)];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kFusedTotPer; ++i) sum += (i < per && t0 + i < nbins) ? tv[i] : 0u;
+    uint32_t total;
+    uint32_t ex = block_scan_waves<kOneThreads>(sum, wsum, total);  // its barriers: rows zeroed
+#pragma unroll
+    for (int i = 0; i < kFusedTotPer; ++i) {
+      if (i < per && t0 + i < nbins) bst[t0 + i] = ex;
+      ex += (i < per && t0 + i < nbins) ? tv[i] : 0u;
+    }
+  } else {
+    __syncthreads();  // rows zeroed
+  }
+  const uint32_t *coff = pr.cnt + (int64_t)chunk * nbins;
+  const uint32_t *bin_start = pr.bin_start;
+  uint32_t key[kOneSteps], rk[kOneSteps], pos[kOneSteps];
   float zr[kOneSteps][3];
   int off[kOneSteps];
 #pragma unroll
@@ -601,6 +664,9 @@ __global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(BinProblem o
     const int64_t j = j0 + 64 * s;
     const bool valid = j < pa.num_points;
     const uint32_t k = valid ? fold_point<REC>(c[s], pr.g, dim, zr[s], off[s]) : 0u;
+    // the chunk's offset (and without the fused scan the tile start): loads
+    // in flight during the ranking
+    pos[s] = coff[k] + (fused ? 0u : bin_start[k]);
     const uint64_t peers = key_peers(k, valid, bits);
     const uint64_t below = peers & lt;
     uint32_t b0 = 0;
@@ -624,17 +690,10 @@ __global__ __launch_bounds__(kOneThreads) void chunk_scatter_kernel(BinProblem o
     }
   }
   __syncthreads();
-  const uint32_t *coff = pr.cnt + (int64_t)chunk * nbins;
-  const uint32_t *bin_start = pr.bin_start;
-  uint32_t pos[kOneSteps];
-  float2 val[kOneSteps];
 #pragma unroll
   for (int s = 0; s < kOneSteps; ++s) {
-    const int64_t j = j0 + 64 * s;
-    const bool valid = j < pa.num_points;
     const uint32_t k = key[s];
-    pos[s] = valid ? bin_start[k] + coff[k] + (uint32_t)row[k] + rk[s] : 0u;
-    val[s] = (REC && pr.values && valid) ? pr.values[j] : make_float2(0.f, 0.f);
+    pos[s] += (fused ? bst[k] : 0u) + (uint32_t)row[k] + rk[s];
   }
 #pragma unroll
   for (int s = 0; s < kOneSteps; ++s) {
@@ -716,27 +775,34 @@ static hipError_t sort_points_one_pass(const PointArgs &pa, const Geom &g, int d
   pr.sub_work = sb.sub_work;
   pr.nchunks = (int)((n + kOneChunk - 1) / kOneChunk);
   const int nch = pr.nchunks;
+  // with the tile starts in LDS beside the count rows, the scatter launch
+  // also writes the list (one more block)
+  pr.fused = n > 0 && nbins <= kFusedMaxBins;
   if (n > 0) {
     hipLaunchKernelGGL(chunk_count_kernel<false>, dim3(nch), dim3(kCountThreads),
                        sizeof(uint32_t) * nbins, s, pr, (const BinProblem *)nullptr, 1);
     hipLaunchKernelGGL(chunk_scan_kernel<false>, dim3((nbins + kScanDigits - 1) / kScanDigits),
                        dim3(kScanDigits * kChunkScanSegs), 0, s, pr, (const BinProblem *)nullptr, 1);
   }
-  // tile starts and the subproblem list need only the counts
-  const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
-  hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
-                     n > 0 ? (const uint32_t *)sb.scan_tmp : (const uint32_t *)nullptr, 1,
-                     sb.nbins, nb, sb.max_sub, sb.sub_work, sb.sub_ofs + sb.nbins, sb.subs);
+  if (!pr.fused) {
+    // tile starts and the subproblem list need only the counts
+    const int3 nb = dim == 3 ? make_int3(g.nbin[0], g.nbin[1], g.nbin[2]) : make_int3(1, 1, 1);
+    hipLaunchKernelGGL(subproblems_kernel, dim3(1), dim3(kSubThreads), 0, s, sb.bin_start,
+                       n > 0 ? (const uint32_t *)sb.scan_tmp : (const uint32_t *)nullptr, 1,
+                       sb.nbins, nb, sb.max_sub, sb.sub_work, sb.sub_ofs + sb.nbins, sb.subs);
+  }
   if (n > 0) {
-    const size_t lds = sizeof(uint16_t) * (size_t)kOneWaves * nbins;
+    size_t lds = sizeof(uint16_t) * (size_t)kOneWaves * nbins;
+    if (pr.fused) lds = std::max(lds + sizeof(uint32_t) * nbins, sizeof(SubLds<kOneThreads>));
     hipError_t e = set_scatter_lds(rec != nullptr, false, lds);
     if (e != hipSuccess) return e;
+    const int blocks = nch + (pr.fused ? 1 : 0);
     if (rec)
-      hipLaunchKernelGGL((chunk_scatter_kernel<true, false>), dim3(nch), dim3(kOneThreads), lds, s,
-                         pr, (const BinProblem *)nullptr, 1);
-    else
-      hipLaunchKernelGGL((chunk_scatter_kernel<false, false>), dim3(nch), dim3(kOneThreads), lds,
+      hipLaunchKernelGGL((chunk_scatter_kernel<true, false>), dim3(blocks), dim3(kOneThreads), lds,
                          s, pr, (const BinProblem *)nullptr, 1);
+    else
+      hipLaunchKernelGGL((chunk_scatter_kernel<false, false>), dim3(blocks), dim3(kOneThreads),
+                         lds, s, pr, (const BinProblem *)nullptr, 1);
   }
   return hipGetLastError();
 }
