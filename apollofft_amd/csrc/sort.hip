@@ -415,6 +415,8 @@ __global__ __launch_bounds__(kSubThreads) void subproblems_kernel(uint32_t *bin_
 //   chunk_scan    : per-tile exclusive scan over the chunks (in place) and
 //                   the tile totals
 //   subproblems   : bin_start = exclusive scan of the totals, and the list
+//                   (up to kFusedMaxBins tiles: block 0 of the scatter
+//                   launch, the chunks then scan the totals themselves)
 //   chunk_scatter : each wave ranks its run's points among equal tiles with
 //                   ballots (earlier lanes, then earlier steps through a
 //                   per-wave LDS row of counts); the rows are scanned across
