@@ -39,12 +39,9 @@ METHODS = ["valu", "mfma"]
 # ---------------------------------------------------------------- binning
 @pytest.mark.parametrize("case", ["uniform", "clustered", "edges"])
 def test_binning_matches_cpu_replay(dev, case):
-    import apollofft_amd._native as N
-    import torch
     rng = np.random.default_rng(11)
     n = (24, 20, 16)
     plan, d = _plan3(n, w=5, policy="exact")
-    meta = plan.meta
     if case == "uniform":
         xyz = uniform_points(rng, 50000)
     elif case == "clustered":
@@ -54,6 +51,30 @@ def test_binning_matches_cpu_replay(dev, case):
                          3 * TWO_PI + 0.1, -5 * TWO_PI - 0.2, np.nextafter(np.float32(np.pi), 0)],
                         dtype=np.float32)
         xyz = rng.choice(vals, size=(3000, 3)).astype(np.float32)
+    _check_binning(dev, plan, n, d, xyz)
+
+
+@pytest.mark.parametrize("n,nbins", [
+    ((105, 176, 176), 7680),   # subproblem list beside the scatter, 150 KiB of LDS
+    ((126, 176, 176), 9216),   # list launch of its own, largest one-pass tile count
+])
+def test_binning_near_lds_limits_matches_cpu_replay(dev, n, nbins):
+    """One-pass binning at tile counts where the scatter's LDS (count rows,
+    tile starts) is close to 160 KiB, on both sides of the fused-list limit
+    (w = 5 matrix-core tiles of 7 x 22 x 22 cells on 2n grids)."""
+    rng = np.random.default_rng(5)
+    plan, d = _plan3(n, w=5, policy="exact", method="mfma")
+    assert tuple(plan.meta["bin_size"]) == (7, 22, 22)
+    xyz = np.concatenate([uniform_points(rng, 150000), clustered_points(rng, 50000)])
+    assert _check_binning(dev, plan, n, d, xyz) == nbins
+
+
+def _check_binning(dev, plan, n, d, xyz):
+    """The device permutation and tile starts equal the CPU replay's; returns
+    the tile count."""
+    import apollofft_amd._native as N
+    import torch
+    meta = plan.meta
     M = xyz.shape[0]
     keys, perm_ref, bs_ref = replay_sort(xyz, n, d, meta["m"], meta["bin_size"])
     nb = ctypes.c_int64()
@@ -70,6 +91,7 @@ def test_binning_matches_cpu_replay(dev, case):
     torch.cuda.synchronize()
     assert np.array_equal(perm.cpu().numpy().view(np.uint32), perm_ref)
     assert np.array_equal(bs.cpu().numpy().view(np.uint32), bs_ref)
+    return nb.value
 
 
 # ---------------------------------------------------------------- spreading
