@@ -516,7 +516,15 @@ void choose_bins(anufft_plan *p) {
   }
   const bool user = p->opts.bin_size[0] > 0 && p->opts.bin_size[1] > 0 && p->opts.bin_size[2] > 0;
   const int cyz = kSpreadCols - 2 * p->w;
-  const int bsp[3] = {spread_bin_x(p->w, p->kernel_method), cyz, cyz};
+  // matrix-core tiles: y and z cut into as many tiles as cyz-cell bins need,
+  // with the cells shared evenly (128 cells: 8 tiles of 16 rather than 7 of
+  // 18 and one of 2), so that no tile pays the fixed cost of a tile for a
+  // sliver of the points
+  auto even = [&](int a) {
+    const int m = (int)p->m[a], nt = (m + cyz - 1) / cyz;
+    return p->kernel_method == kSpreadMfma ? (m + nt - 1) / nt : cyz;
+  };
+  const int bsp[3] = {spread_bin_x(p->w, p->kernel_method), even(1), even(2)};
   set_geom(p->bs, p, bsp);
   p->bs.max_sub = p->opts.max_subproblem > 0 ? p->opts.max_subproblem : 4096;
 

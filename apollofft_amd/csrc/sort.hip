@@ -289,7 +289,7 @@ __device__ __forceinline__ uint64_t block_scan_waves64(uint64_t v, uint64_t *sh,
 // subproblems each, exclusive scan of those counts in rounds of kSubRound
 // tiles (kSubPer consecutive tiles per thread).  subs[i].w = work before
 // subproblem i in list order, in units of kWorkUnit points, a subproblem
-// weighing ceil(count / kWorkUnit) + kSubWork (its fixed cost); nsub[0] =
+// weighing exactly ceil(count / kWorkUnit) + sub_work (its fixed cost); nsub[0] =
 // number of subproblems, nsub[1] = total work.  The subproblem counts and
 // the work are scanned together (one 64-bit scan: counts in the high word).
 // The entries of a round are written by the whole block, each finding its
@@ -358,7 +358,10 @@ __device__ __forceinline__ void subproblems_body(SubLds<NT> &L, uint32_t *bin_st
         start[i] = bin_start[b[i]];
         c[i] = bin_start[b[i] + 1] - start[i];
         k[i] = (c[i] + ms - 1) / ms;
-        wk[i] = (c[i] + kWorkUnit - 1) / kWorkUnit + k[i] * sw;
+        // k - 1 full subproblems and the rest: each interval of the work
+        // prefix is its fixed cost plus its own units, for any max_sub
+        const uint32_t rest = k[i] > 0 ? c[i] - (k[i] - 1) * ms : 0u;
+        wk[i] = k[i] > 0 ? (k[i] - 1) * wfull + (rest + kWorkUnit - 1) / kWorkUnit + sw : 0u;
       }
       sum += ((uint64_t)k[i] << 32) | wk[i];
     }

@@ -774,7 +774,13 @@ __device__ __forceinline__ void mfma_rescale(f32x16 (&acc)[MSpread<W>::XW][2], f
 // block b on XCD b mod 8) and splits it evenly among its blocks, so a
 // workgroup walks consecutive tiles of a pencil and the XCD's tiles meet in
 // its L2.
-__device__ __forceinline__ void xcd_work_range(uint32_t wt, uint32_t &w0, uint32_t &w1) {
+//
+// Weights: blocks of the first half of the grid take wa shares, the others
+// wb.  With two workgroups per CU the older one (dealt first) is issued
+// ahead of the younger and runs faster, so equal shares leave the younger
+// half finishing last; wa > wb hands the older half correspondingly more.
+__device__ __forceinline__ void xcd_work_range(uint32_t wt, uint32_t &w0, uint32_t &w1,
+                                               uint32_t wa = 1, uint32_t wb = 1) {
   constexpr uint32_t kXcd = 8;
   const uint32_t g = gridDim.x, b = blockIdx.x;
   if (g < kXcd) {
@@ -784,9 +790,15 @@ __device__ __forceinline__ void xcd_work_range(uint32_t wt, uint32_t &w0, uint32
   }
   const uint32_t x = b % kXcd, j = b / kXcd;
   const uint32_t per = (g - x + kXcd - 1) / kXcd;  // blocks on XCD x
+  const uint32_t half = g / 2;                     // blocks j < nh of XCD x are in the first half
+  const uint32_t nh = min(per, half > x ? (half - x + kXcd - 1) / kXcd : 0u);
+  auto before = [&](uint32_t k) {  // shares of the XCD's blocks 0 .. k - 1
+    return (uint64_t)k * wb + (uint64_t)min(k, nh) * (wa - wb);
+  };
+  const uint64_t tot = before(per);
   const uint64_t lo = (uint64_t)wt * x / kXcd, hi = (uint64_t)wt * (x + 1) / kXcd;
-  w0 = (uint32_t)(lo + (hi - lo) * j / per);
-  w1 = (uint32_t)(lo + (hi - lo) * (j + 1) / per);
+  w0 = (uint32_t)(lo + (hi - lo) * before(j) / tot);
+  w1 = (uint32_t)(lo + (hi - lo) * before(j + 1) / tot);
 }
 
 // First subproblem i in [0, n) whose work prefix subs[i].w is >= w (n if none).
@@ -800,27 +812,93 @@ __device__ __forceinline__ uint32_t work_lower_bound(const int4 *subs, uint32_t 
   return lo;
 }
 
-// Contiguous run of the subproblem list for this workgroup, balanced by the
-// work prefix of the list (points plus a fixed cost per subproblem).
+// Shares of the older and younger workgroup of a CU (two per CU) in the
+// spreading and interpolation launches (measured at 2e6 points, w = 7: the
+// younger interpolation workgroups ran about 12 % longer on equal shares,
+// 8 : 7 cut the launch by 4 %; spreading did not respond to weights).
+constexpr uint32_t kSpreadOlder = 1, kSpreadYounger = 1;
+constexpr uint32_t kInterpOlder = 8, kInterpYounger = 7;
+
+// The share of the subproblem list of this workgroup: the work range
+// [w0, w1) of xcd_work_range over the list's work prefix.  Subproblem i
+// spans the work interval [W_i, W_i+1): its fixed cost first, then one unit
+// per kWorkUnit points.  The workgroup owns the points whose units fall in
+// its range, so the subproblems at the two ends of its run may be split with
+// the neighbouring workgroups (each pays the fixed cost) and every
+// workgroup gets the same work however coarse the subproblems are; every
+// unit lies in exactly one range, so the shares partition the points.
+// begin .. end - 1 are the subproblems whose intervals meet the range;
+// points [h0, h1) of subproblem begin and [0, t1) of subproblem end - 1 are
+// this workgroup's, every subproblem between is whole.
 struct XcdChunk {
   uint32_t begin, end;
-  __device__ __forceinline__ XcdChunk(const PointArgs &pa) {
-    const uint32_t n = pa.nsub[0];
+  int h0, h1, t1;
+  __device__ __forceinline__ XcdChunk(const PointArgs &pa, uint32_t wa = 1, uint32_t wb = 1) {
     uint32_t w0, w1;
-    xcd_work_range(pa.nsub[1], w0, w1);
-    begin = work_lower_bound(pa.subs, n, w0);
-    end = work_lower_bound(pa.subs, n, w1);
+    xcd_work_range(pa.nsub[1], w0, w1, wa, wb);
+    init(pa.subs, pa.nsub[0], pa.nsub[1], w0, w1);
+  }
+  // The share [w0, w1) of a list of n subproblems of total work wt.
+  __device__ __forceinline__ XcdChunk(const int4 *subs, uint32_t n, uint32_t wt, uint32_t w0,
+                                      uint32_t w1) {
+    init(subs, n, wt, w0, w1);
+  }
+  __device__ __forceinline__ void init(const int4 *subs, uint32_t n, uint32_t wt, uint32_t w0,
+                                       uint32_t w1) {
+    const uint32_t b = work_lower_bound(subs, n, w0 + 1);  // first i with W_i > w0
+    begin = b > 0 ? b - 1 : 0;
+    end = work_lower_bound(subs, n, w1);
+    // drop a head or tail subproblem of which no point is this workgroup's
+    // (the range starts past its points, or ends inside its fixed cost), so
+    // that every subproblem of the run has points
+    h0 = h1 = t1 = 0;
+    int t0;
+    if (begin < end) {
+      owned(subs, n, wt, begin, w0, w1, h0, h1);
+      if (h1 <= h0 && ++begin < end) owned(subs, n, wt, begin, w0, w1, h0, h1);
+      if (h1 <= h0) begin = end;
+    }
+    if (end - begin > 1) {
+      owned(subs, n, wt, end - 1, w0, w1, t0, t1);
+      if (t1 <= 0 && --end - begin > 1) owned(subs, n, wt, end - 1, w0, w1, t0, t1);
+    }
+    begin = __builtin_amdgcn_readfirstlane(begin);
+    end = __builtin_amdgcn_readfirstlane(end);
+    h0 = __builtin_amdgcn_readfirstlane(h0);
+    h1 = __builtin_amdgcn_readfirstlane(h1);
+    t1 = __builtin_amdgcn_readfirstlane(t1);
+  }
+  // Points [p0, p1) of subproblem i whose units fall in [w0, w1).
+  __device__ __forceinline__ static void owned(const int4 *subs, uint32_t n, uint32_t wt, uint32_t i,
+                                               uint32_t w0, uint32_t w1, int &p0, int &p1) {
+    const int4 sub = subs[i];
+    const uint32_t wi = (uint32_t)sub.w, wn = i + 1 < n ? (uint32_t)subs[i + 1].w : wt;
+    const uint32_t u = ((uint32_t)sub.z + kWorkUnit - 1) / kWorkUnit;
+    const uint32_t a = wi + (wn - wi > u ? wn - wi - u : 0u);  // unit 0's place
+    const uint32_t last = wn - 1;  // unit k sits at min(a + k, last)
+    auto first = [&](uint32_t w) { return last < w ? u : (w <= a ? 0u : min(u, w - a)); };
+    p0 = (int)(first(w0) * kWorkUnit);
+    p1 = min(sub.z, (int)(first(w1) * kWorkUnit));
+  }
+  // Subproblem i (sub = subs[i]) of begin .. end - 1 narrowed to this
+  // workgroup's points (at least one).
+  __device__ __forceinline__ int4 part(uint32_t i, int4 sub) const {
+    const int y = i == begin ? sub.y + h0 : sub.y;
+    const int z = i == begin ? h1 - h0 : (i + 1 == end ? t1 : sub.z);
+    // wave-uniform values: kept in scalar registers
+    return make_int4(sub.x, __builtin_amdgcn_readfirstlane(y), __builtin_amdgcn_readfirstlane(z),
+                     sub.w);
   }
 };
 
 // Spread the subproblems [begin, end) of one problem's list with one ring
 // window: a tile further along the pencil keeps the columns both cover;
 // everything left is flushed at the end.
-template <int W>
+template <int W, class Part>
 __device__ __forceinline__ void spread_run_mfma(const int4 *__restrict__ subs, uint32_t begin,
-                                                uint32_t end, const ZRec &zr, float2 *grid,
-                                                const Geom &g, const KbPoly *__restrict__ dpoly,
-                                                char *shm) {
+                                                uint32_t end, const Part &part, const ZRec &zr,
+                                                float2 *grid, const Geom &g,
+                                                const KbPoly *__restrict__ dpoly, char *shm) {
   using MS = MSpread<W>;
   float *red = (float *)(shm + MLds<MS::PX>::RED);
   f32x16 acc[MS::XW][2];
@@ -831,7 +909,7 @@ __device__ __forceinline__ void spread_run_mfma(const int4 *__restrict__ subs, u
   MWindow win;
   int chain = 0;  // points accumulated since the last full flush
   for (uint32_t sidx = begin; sidx < end; ++sidx) {
-    const int4 sub = subs[sidx];
+    const int4 sub = part.part(sidx, subs[sidx]);
     const MTile t = mfma_tile<W>(g, sub.x);
     const int e = mfma_strength_exp<MS::NW>(zr, sub, red);
     if (win.prob >= 0) {
@@ -859,8 +937,8 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
   using MS = MSpread<W>;
   extern __shared__ __align__(16) char shm[];
   stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
-  const XcdChunk r(pa);
-  spread_run_mfma<W>(pa.subs, r.begin, r.end, zr, grid, g, dpoly, shm);
+  const XcdChunk r(pa, kSpreadOlder, kSpreadYounger);
+  spread_run_mfma<W>(pa.subs, r.begin, r.end, r, zr, grid, g, dpoly, shm);
 }
 
 // Batched form: one persistent launch over the subproblems of all problems
@@ -887,19 +965,16 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_batched_kerne
     }
     return lo;
   };
-  auto global_lb = [&](uint32_t w) -> uint32_t {
-    if (w >= wpre[nprob]) return prefix[nprob];
-    const int p = problem_of(w);
-    return prefix[p] + work_lower_bound(probs[p].pa.subs, prefix[p + 1] - prefix[p], w - wpre[p]);
-  };
-  const uint32_t gbegin = global_lb(w0), gend = global_lb(w1);
-  if (gbegin >= gend) return;
-  for (int p = problem_of(w0); p < nprob && prefix[p] < gend; ++p) {
-    const uint32_t lb = max(gbegin, prefix[p]) - prefix[p];
-    const uint32_t le = min(gend, prefix[p + 1]) - prefix[p];
-    if (lb >= le) continue;
+  if (w0 >= w1) return;
+  // each problem the range meets: its share of the problem's own list, as
+  // for a single problem (subproblems at the range ends split)
+  for (int p = problem_of(w0); p < nprob && wpre[p] < w1; ++p) {
+    const uint32_t lo = wpre[p], hi = wpre[p + 1];
+    if (hi <= w0) continue;  // no work of this problem in the range
     const SpreadProblem &pr = probs[p];
-    spread_run_mfma<W>(pr.pa.subs, lb, le, pr.zr, pr.grid, pr.g, dpoly, shm);
+    const XcdChunk part(pr.pa.subs, prefix[p + 1] - prefix[p], hi - lo, max(w0, lo) - lo,
+                        min(w1, hi) - lo);
+    spread_run_mfma<W>(pr.pa.subs, part.begin, part.end, part, pr.zr, pr.grid, pr.g, dpoly, shm);
   }
 }
 
@@ -1220,9 +1295,9 @@ __global__ __launch_bounds__(MSpread<W>::NT, win_groups_per_cu<W>()) void interp
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, g4 = lane >> 4;
-  const XcdChunk xr(pa);
+  const XcdChunk xr(pa, kInterpOlder, kInterpYounger);
   for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
-    const int4 sub = pa.subs[sidx];
+    const int4 sub = xr.part(sidx, pa.subs[sidx]);
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
     const int by = (bin / g.nbin[2]) % g.nbin[1];
@@ -1523,7 +1598,7 @@ __global__ __launch_bounds__(MGrad<W>::NT, 1) void grad3d_win_kernel(
   const int i16 = lane & 15, g4 = lane >> 4;
   const XcdChunk xr(pa);
   for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
-    const int4 sub = pa.subs[sidx];
+    const int4 sub = xr.part(sidx, pa.subs[sidx]);
     const int bin = sub.x;
     const int bz = bin % g.nbin[2];
     const int by = (bin / g.nbin[2]) % g.nbin[1];

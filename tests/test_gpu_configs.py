@@ -219,7 +219,10 @@ def test_batched_64_problems_radial(dev, oracle_mod):
                                          (nn,) * 3, (TWO_PI / nn,) * 3, sel)
         errs.append(rel_l2(outs[i].cpu().numpy().ravel()[sel], ref))
     _say(f"batched: max rel diff vs per-problem {worst:.3e}, direct {max(errs):.3e}")
-    assert worst < 1e-6
+    # two FP32 evaluations of the same sums with different work splits (so
+    # different subproblem parts and strength scales), each within about 1e-6
+    # of the FP64 sums, agree to within twice that
+    assert worst < 2e-6
     assert max(errs) < TOL
 
     # the batched call captured in a HIP graph, replayed with eager calls and
