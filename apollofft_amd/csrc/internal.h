@@ -243,9 +243,11 @@ hipError_t launch_spread3d_batched(int w, int method, const SpreadProblem *host_
 // last FFT pass (FftPass::bmax), so only the tile maxima are computed.
 hipError_t launch_tile_max3d(int w, int method, const float2 *grid, float *scratch, const Geom &g,
                              hipStream_t s, bool bin_max_done = false);
+// cus: the device's CU count (the windowed kernel weights the shares of the
+// older and younger workgroup of a CU when the grid holds two per CU).
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
-                           const KbPoly *dpoly, int num_blocks, hipStream_t s);
+                           const KbPoly *dpoly, int num_blocks, int cus, hipStream_t s);
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                          const float2 *u, float *gx, float *gy, float *gz, const Geom &g,
                          const KbPoly &poly, const KbPoly *dpoly, int num_blocks, hipStream_t s);

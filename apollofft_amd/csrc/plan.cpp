@@ -1179,7 +1179,8 @@ anufft_status anufft_execute_type2(const anufft_plan *p, int64_t M, const float 
     e = ex.timed(ANUFFT_STAGE_INTERP, [&] {
       return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(),
                              (float *)(ex.ws + lay.tmax), (float2 *)out, p->bi.g, p->poly, p->d_poly,
-                             ex.blocks(p->bi, M, interp_blocks_per_cu(p->w, p->kernel_method)), ex.s);
+                             ex.blocks(p->bi, M, interp_blocks_per_cu(p->w, p->kernel_method)),
+                             p->cus, ex.s);
     });
   } else {
     PointArgs pa = ex.point_args(p->bi, M, coords, stride);
@@ -1328,7 +1329,7 @@ anufft_status anufft_execute_type1_backward(const anufft_plan *p, int64_t M,
     return launch_interp3d(p->w, p->kernel_method, pa, ex.zrec(M), ex.grid(),
                            (float *)(ex.ws + lay.tmax), (float2 *)grad_values, p->bi.g, p->poly,
                            p->d_poly, ex.blocks(p->bi, M, interp_blocks_per_cu(p->w, p->kernel_method)),
-                           ex.s);
+                           p->cus, ex.s);
   });
   if (e != hipSuccess) return hip_fail(e, "interp");
   e = ex.timed(ANUFFT_STAGE_GRAD, [&] {

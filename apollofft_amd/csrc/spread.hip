@@ -812,11 +812,11 @@ __device__ __forceinline__ uint32_t work_lower_bound(const int4 *subs, uint32_t 
   return lo;
 }
 
-// Shares of the older and younger workgroup of a CU (two per CU) in the
-// spreading and interpolation launches (measured at 2e6 points, w = 7: the
-// younger interpolation workgroups ran about 12 % longer on equal shares,
-// 8 : 7 cut the launch by 4 %; spreading did not respond to weights).
-constexpr uint32_t kSpreadOlder = 1, kSpreadYounger = 1;
+// Shares of the older and younger workgroup of a CU in the interpolation
+// launch when it holds two per CU (measured at 2e6 points, w = 7: the
+// younger workgroups ran about 12 % longer on equal shares, 8 : 7 cut the
+// launch by 4 %; spreading, bound by its matrix work per CU, did not
+// respond to weights and keeps equal shares).
 constexpr uint32_t kInterpOlder = 8, kInterpYounger = 7;
 
 // The share of the subproblem list of this workgroup: the work range
@@ -937,7 +937,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, 2) void spread3d_mfma_kernel(
   using MS = MSpread<W>;
   extern __shared__ __align__(16) char shm[];
   stage_poly_rows(dpoly->c, 2 * W, (float *)(shm + MLds<MS::PX>::POLY));  // ordered by the first barrier
-  const XcdChunk r(pa, kSpreadOlder, kSpreadYounger);
+  const XcdChunk r(pa);
   spread_run_mfma<W>(pa.subs, r.begin, r.end, r, zr, grid, g, dpoly, shm);
 }
 
@@ -1270,7 +1270,7 @@ constexpr int win_groups_per_cu() { return W <= 7 ? 2 : 1; }
 template <int W>
 __global__ __launch_bounds__(MSpread<W>::NT, win_groups_per_cu<W>()) void interp3d_win_kernel(
     PointArgs pa, ZRec zr, const float2 *grid, const float *tmax, float2 *out, Geom g,
-    const KbPoly *__restrict__ dpoly) {
+    const KbPoly *__restrict__ dpoly, uint32_t wa, uint32_t wb) {
   using MS = MSpread<W>;
   using MW = MWin<W>;
   constexpr int NT = MS::NT, XW = MS::XW, NW = MS::NW, PX = MS::PX, NCLS = MW::NCLS;
@@ -1295,7 +1295,7 @@ __global__ __launch_bounds__(MSpread<W>::NT, win_groups_per_cu<W>()) void interp
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i16 = lane & 15, g4 = lane >> 4;
-  const XcdChunk xr(pa, kInterpOlder, kInterpYounger);
+  const XcdChunk xr(pa, wa, wb);
   for (uint32_t sidx = xr.begin; sidx < xr.end; ++sidx) {
     const int4 sub = xr.part(sidx, pa.subs[sidx]);
     const int bin = sub.x;
@@ -2240,15 +2240,19 @@ struct TileMax3 {
 template <int W>
 struct Interp3Mfma {
   static hipError_t run(const PointArgs &pa, ZRec zr, const float2 *grid, float *bmax,
-                        float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, hipStream_t s) {
+                        float2 *out, const Geom &g, const KbPoly *dpoly, int nblk, int cus,
+                        hipStream_t s) {
     const float *tmax = bmax + g.nbin[0] * g.nbin[1] * g.nbin[2];  // from TileMax3
+    // older / younger shares only when the grid holds two workgroups per CU
+    const bool two = win_groups_per_cu<W>() == 2 && nblk == 2 * cus;
+    const uint32_t wa = two ? kInterpOlder : 1u, wb = two ? kInterpYounger : 1u;
     // 16-row windows hold the 2w tap rows of a class of points
     constexpr int lds = JLds<MSpread<W>::PX, MSpread<W>::NW>::SIZE;
     hipError_t e = hipFuncSetAttribute((const void *)interp3d_win_kernel<W>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(interp3d_win_kernel<W>, dim3(nblk), dim3(MSpread<W>::NT), lds, s, pa, zr,
-                       grid, (const float *)tmax, out, g, dpoly);
+                       grid, (const float *)tmax, out, g, dpoly, wa, wb);
     return hipGetLastError();
   }
 };
@@ -2395,9 +2399,9 @@ hipError_t launch_tile_max3d(int w, int method, const float2 *grid, float *scrat
 }
 hipError_t launch_interp3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
                            float *scratch, float2 *out, const Geom &g, const KbPoly &poly,
-                           const KbPoly *dpoly, int nblk, hipStream_t s) {
+                           const KbPoly *dpoly, int nblk, int cus, hipStream_t s) {
   if (method == kSpreadMfma)
-    return dispatch_w<Interp3Mfma>(w, pa, zr, grid, scratch, out, g, dpoly, nblk, s);
+    return dispatch_w<Interp3Mfma>(w, pa, zr, grid, scratch, out, g, dpoly, nblk, cus, s);
   return dispatch_w<Interp3>(w, pa, grid, out, g, poly, nblk, s);
 }
 hipError_t launch_grad3d(int w, int method, const PointArgs &pa, ZRec zr, const float2 *grid,
