@@ -793,7 +793,8 @@ __device__ __forceinline__ void xcd_work_range(uint32_t wt, uint32_t &w0, uint32
   const uint32_t half = g / 2;                     // blocks j < nh of XCD x are in the first half
   const uint32_t nh = min(per, half > x ? (half - x + kXcd - 1) / kXcd : 0u);
   auto before = [&](uint32_t k) {  // shares of the XCD's blocks 0 .. k - 1
-    return (uint64_t)k * wb + (uint64_t)min(k, nh) * (wa - wb);
+    const uint32_t m = min(k, nh);
+    return (uint64_t)m * wa + (uint64_t)(k - m) * wb;
   };
   const uint64_t tot = before(per);
   const uint64_t lo = (uint64_t)wt * x / kXcd, hi = (uint64_t)wt * (x + 1) / kXcd;
@@ -820,9 +821,9 @@ __device__ __forceinline__ uint32_t work_lower_bound(const int4 *subs, uint32_t 
 constexpr uint32_t kInterpOlder = 8, kInterpYounger = 7;
 
 // The share of the subproblem list of this workgroup: the work range
-// [w0, w1) of xcd_work_range over the list's work prefix.  Subproblem i
-// spans the work interval [W_i, W_i+1): its fixed cost first, then one unit
-// per kWorkUnit points.  The workgroup owns the points whose units fall in
+// [w0, w1) of xcd_work_range over the list's work prefix W (W(i) =
+// subs[i].w).  Subproblem i spans the work interval [W(i), W(i + 1)): its
+// fixed cost first, then one unit per kWorkUnit points.  The workgroup owns the points whose units fall in
 // its range, so the subproblems at the two ends of its run may be split with
 // the neighbouring workgroups (each pays the fixed cost) and every
 // workgroup gets the same work however coarse the subproblems are; every
@@ -845,7 +846,7 @@ struct XcdChunk {
   }
   __device__ __forceinline__ void init(const int4 *subs, uint32_t n, uint32_t wt, uint32_t w0,
                                        uint32_t w1) {
-    const uint32_t b = work_lower_bound(subs, n, w0 + 1);  // first i with W_i > w0
+    const uint32_t b = work_lower_bound(subs, n, w0 + 1);  // first i with W(i) > w0
     begin = b > 0 ? b - 1 : 0;
     end = work_lower_bound(subs, n, w1);
     // drop a head or tail subproblem of which no point is this workgroup's
