@@ -336,6 +336,34 @@ def test_type1_clustered_and_tiny_subproblems(dev, oracle_mod, method):
     assert rel_l2(out, ref) < TOL
 
 
+@pytest.mark.parametrize("max_sub,sub_work,per_cu", [(100, 0, 2), (1000, 1, 3), (16384, 384, 1)])
+def test_work_shares_with_odd_settings(dev, oracle_mod, max_sub, sub_work, per_cu):
+    """The persistent launches split subproblems at the boundaries of the
+    workgroups' work shares; with an odd max_subproblem, no or a large fixed
+    cost per subproblem and one to three workgroups per CU the shares still
+    cover every point once (Type-1 and Type-2 against FP64 direct sums)."""
+    import apollofft_amd as A
+    rng = np.random.default_rng(17)
+    n, w = (20, 16, 24), 6
+    d = [TWO_PI / k for k in n]
+    plan = A.NufftPlan3D(*n, *d, 2, w, kernel_method="mfma", max_subproblem=max_sub,
+                         sub_work=sub_work, blocks_per_cu=per_cu)
+    M = 30000
+    xyz = clustered_points(rng, M)
+    c = cnormal(rng, M)
+    f = cnormal(rng, n)
+    tx = _t(xyz, dev)
+    x64 = xyz.astype(np.float64)
+    out1 = plan.type1(tx, _t(c, dev)).cpu().numpy()
+    sel = rng.choice(int(np.prod(n)), size=512, replace=False).astype(np.int64)
+    ref1 = oracle_mod.direct_type1_3d(x64, c.astype(np.complex128), n, d, sel)
+    assert rel_l2(out1.ravel()[sel], ref1) < TOL
+    out2 = plan.type2(tx, _t(f, dev)).cpu().numpy()
+    pj = rng.choice(M, size=500, replace=False)
+    ref2 = oracle_mod.direct_type2_3d(f.astype(np.complex128), x64[pj], d)
+    assert rel_l2(out2[pj], ref2) < TOL
+
+
 @pytest.mark.parametrize("method", METHODS)
 @pytest.mark.parametrize("scale", [1e-30, 1e-6, 1.0, 1e8, 1e30])
 def test_type1_strength_range(dev, oracle_mod, method, scale):
