@@ -299,7 +299,9 @@ __global__ __launch_bounds__(kFftThreads) void fft_pass_kernel(FftPass p) {
   const uint32_t mag_nl = nl == lb ? p.mag_lb : p.mag_last;  // nl < lb only in the last block
   // element offsets of the block's lines, one 64-bit division per line
   for (int li = threadIdx.x; li < nl; li += kFftThreads) {
-    const int64_t l = l0 + li, a = l / p.nb, b = l - a * p.nb;
+    const int64_t l = l0 + li;
+    const int64_t a = p.use_mag_nb ? (int64_t)fdiv((int)l, (int)p.nb, p.mag_nb) : l / p.nb;
+    const int64_t b = l - a * p.nb;
     lbase[li] = a * p.in_sa + b * p.in_sb;
     lbase[lb + li] = a * p.out_sa + b * p.out_sb;
   }
@@ -390,6 +392,9 @@ hipError_t launch_fft_pass(const FftPass &pass, hipStream_t s) {
   p.mag_bmz = div_magic(p.bmax ? p.bm_bin[2] : 1);
   p.mag_lb = div_magic(p.lb);
   p.mag_last = div_magic(p.nlines - (p.nlines - 1) / p.lb * p.lb);
+  p.use_mag_nb = p.nlines < ((int64_t)1 << 31) &&
+                         (uint64_t)p.nlines * (uint64_t)p.nb < ((uint64_t)1 << 32) ? 1 : 0;
+  p.mag_nb = div_magic(p.nb);
   {
     const int Lf0 = p.blue_n > 0 ? p.blue_n : p.L;
     int64_t ns = 1;

@@ -126,8 +126,10 @@ struct FftPass {
   // z bin width of the bin maxima, the lines of a full and of the last
   // block, and per stage f the butterflies per line Lf / fac[f] and the
   // stage span Ns (product of fac[0 .. f-1]); the twiddle step of stage f,
-  // Lf / (Ns fac[f]).
-  uint32_t mag_nin, mag_nout, mag_bmz, mag_lb, mag_last;
+  // Lf / (Ns fac[f]).  mag_nb: the line index split l / nb, used when
+  // nlines * nb < 2^32 (use_mag_nb), else the kernel divides in 64 bits.
+  uint32_t mag_nin, mag_nout, mag_bmz, mag_lb, mag_last, mag_nb;
+  int use_mag_nb;
   uint32_t mag_nbf[16], mag_ns[16];
   int twstep[16];
 };
